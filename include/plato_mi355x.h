@@ -1,0 +1,170 @@
+/*
+ * plato_mi355x.h -- C ABI of the MI355X (gfx950) Tet4 SIMP elastostatics library.
+ *
+ * One handle holds one mesh, one isotropic material, one MSIMP penalty, its essential
+ * and natural boundary conditions and the device data derived from them (block graph,
+ * slot codes, LDS chunks).  All compute entry points take DEVICE pointers (HBM) and run
+ * on the handle's HIP stream; set-up entry points take HOST pointers.  Every entry
+ * point returns PA_OK (0) or a non-zero status; pa_last_error() gives the message of
+ * the calling thread's last failure.  A handle is not thread-safe: one thread (and one
+ * stream) per handle, as the reference managers are single-threaded on the host.
+ *
+ * Layouts follow the reference (src/PlatoTypes.hpp:15-24, LayoutRight):
+ *   coords  f64[3*num_nodes]  x,y,z per node
+ *   conn    i32[4*num_elems]  Tet4 vertices per element
+ *   u, R    f64[3*num_nodes]  dof 3*n+i
+ *   z       f64[num_nodes]    nodal density (control)
+ *   Jacobian values f64[9*nnz_b], block (slot s, i, j) at 9*s+3*i+j
+ *   gradient_z values f64[3*nnz_b], transposed 1x3 blocks, entry 3*s+i
+ */
+#ifndef PLATO_MI355X_H
+#define PLATO_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pa_status {
+  PA_OK = 0,
+  PA_ERR_ARG = 1,          /* bad argument (null pointer, size, index) */
+  PA_ERR_HIP = 2,          /* HIP runtime error (allocation, launch)    */
+  PA_ERR_MESH = 3,         /* mesh/graph inconsistent                   */
+  PA_ERR_STATE = 4,        /* call order (e.g. compute before finalize) */
+  PA_ERR_NOT_CONVERGED = 5,/* linear solver hit its iteration cap       */
+  PA_ERR_UNSUPPORTED = 6   /* mesh outside the kernels' limits          */
+};
+
+typedef struct pa_problem* pa_handle;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* pa_last_error(void);
+
+/* ---------------------------------------------------------------- mesh utilities (host)
+ * Box [0,lx]x[0,ly]x[0,lz] with nx*ny*nz hexes split into 6 Tet4 each, numbered like
+ * Omega_h::build_box (used by the reference tests: unit_tests/PlatoTestHelpers.hpp:206-250).
+ * Call with coords==NULL/conn==NULL to get the sizes, then again with buffers of
+ * 3*num_nodes doubles and 4*num_elems ints. */
+int pa_build_box(int nx, int ny, int nz, double lx, double ly, double lz, int* num_nodes,
+                 int* num_elems, double* coords, int32_t* conn);
+
+/* Node set of a coordinate plane (mesh "node set", src/EssentialBCs.hpp:185-218 consumes
+ * it).  out==NULL returns the count only. */
+int pa_nodes_on_plane(int num_nodes, const double* coords, int axis, double value, double tol,
+                      int32_t* out, int* count);
+
+/* Side set of a coordinate plane as face vertex triples plus the adjacent element
+ * (what src/SurfaceLoadIntegral.hpp:100-157 walks).  NULL outputs return the count. */
+int pa_faces_on_plane(int num_elems, const int32_t* conn, const double* coords, int axis,
+                      double value, double tol, int32_t* face_verts, int32_t* face_elems,
+                      int* count);
+
+/* Block graph of a mesh on the host: node star plus diagonal, ascending columns
+ * (CreateBlockMatrix, src/ImplicitFunctors.hpp:735-808).  colidx==NULL returns
+ * rowmap (num_nodes+1 entries) and *nnz_blocks only. */
+int pa_node_graph(int num_nodes, int num_elems, const int32_t* conn, int32_t* rowmap, int32_t* colidx,
+                  int* nnz_blocks);
+
+/* ---------------------------------------------------------------- problem set-up (host)
+ * Mesh arrays are copied; the caller keeps ownership. */
+int pa_create(pa_handle* h, int num_nodes, int num_elems, const double* coords,
+              const int32_t* conn);
+int pa_destroy(pa_handle h);
+
+/* "Isotropic Linear Elastic" material: Youngs Modulus, Poissons Ratio
+ * (src/IsotropicLinearElasticMaterial.cpp:87-127). */
+int pa_set_material(pa_handle h, double youngs_modulus, double poissons_ratio);
+
+/* MSIMP penalty: "Exponent" p, "Minimum Value" rho_min (src/Simp.hpp:44-50,130-134):
+ * w = rho_min + (1-rho_min)*pow(mean(z_e), p).  Defaults 3 and 1e-9 as the reference. */
+int pa_set_penalty(pa_handle h, double exponent, double minimum_value);
+
+/* Essential BC on one dof of a node set (src/EssentialBCs.hpp:185-218), value scaled like
+ * the reference's step scale in applyBlockConstraints. */
+int pa_add_essential_bc(pa_handle h, int num_nodes, const int32_t* nodes, int dof, double value);
+
+/* Uniform traction on a side set (src/NaturalBC.hpp:186-228, SurfaceLoadIntegral):
+ * each face vertex receives area/3 * traction in f, and R = K u - f. */
+int pa_add_traction(pa_handle h, int num_faces, const int32_t* face_verts,
+                    const int32_t* face_elems, const double traction[3]);
+
+/* Optional caller block graph (rowmap i32[num_nodes+1], colidx i32[nnz_b]); without it
+ * the node star plus diagonal is built with ascending columns
+ * (CreateBlockMatrix, src/ImplicitFunctors.hpp:735-808).  Assembly writes values in the
+ * order of whichever graph the handle holds, so a reference graph is reproduced exactly. */
+int pa_set_graph(pa_handle h, const int32_t* rowmap, const int32_t* colidx);
+
+/* Builds the device data (graph, slot codes, chunks, loads, BC lists).  Must be called
+ * once after the set-up calls and before any compute call. */
+int pa_finalize(pa_handle h);
+
+/* HIP stream for all later launches (hipStream_t passed as void*; NULL = default). */
+int pa_set_stream(pa_handle h, void* hip_stream);
+
+int pa_graph_size(pa_handle h, int* num_rows, int* nnz_blocks);
+/* Copies the graph out; on_device selects device (1) or host (0) destination buffers. */
+int pa_copy_graph(pa_handle h, int32_t* rowmap, int32_t* colidx, int on_device);
+
+/* ------------------------------------------------ Plato::Elliptic::VectorFunction (device)
+ * value: R(u,z) = K(z) u - f          (src/elliptic/VectorFunction.hpp:231-313)
+ * gradient_u: K(z) into values, optionally R in the same pass (R may be NULL)
+ *                                      (src/elliptic/VectorFunction.hpp:505-594; the
+ *                                      gradient_u_T of :412 is the same matrix here)
+ * gradient_z: dR/dz as transposed 1x3 blocks on the same graph
+ *                                      (src/elliptic/VectorFunction.hpp:598-689) */
+int pa_vf_value(pa_handle h, const double* u, const double* z, double* R);
+int pa_vf_gradient_u(pa_handle h, const double* u, const double* z, double* values, double* R);
+int pa_vf_gradient_z(pa_handle h, const double* u, const double* z, double* values_t);
+
+/* ---------------------------------- Plato::Elliptic::PhysicsScalarFunction, criterion
+ * "Internal Elastic Energy" (src/elliptic/InternalElasticEnergy.hpp:100-158):
+ * value C = sum_e 0.5*vol*sigma_w:eps   (PhysicsScalarFunction.hpp:225-283), value to host
+ * gradient_u = K(z) u                   (PhysicsScalarFunction.hpp:365-…)
+ * gradient_z = partial dC/dz (nodal)    (PhysicsScalarFunction.hpp:432-495) */
+int pa_sf_value(pa_handle h, const double* u, const double* z, double* value);
+int pa_sf_gradient_u(pa_handle h, const double* u, const double* z, double* g);
+int pa_sf_gradient_z(pa_handle h, const double* u, const double* z, double* g, double* value);
+
+/* ------------------------------------------------------------ Problem / solver (device) */
+typedef struct {
+  double tolerance;   /* relative residual ||b-Ax||/||b|| to stop at                 */
+  int max_iters;      /* iteration cap (PA_ERR_NOT_CONVERGED when reached)           */
+  int check_every;    /* host convergence check period (iterations)                  */
+  int precision;      /* 0: FP64 block-CRS PCG; 1: FP32 matrix-free PCG + FP64 refinement */
+  int use_x0;         /* 1: start from the contents of x (warm start), 0: from zero  */
+  double inner_tolerance; /* FP32 inner solve relative tolerance (precision 1)       */
+} pa_solver_opts;
+
+typedef struct {
+  int iterations;       /* total inner iterations                                    */
+  int outer_iterations; /* refinement sweeps (precision 1)                           */
+  int converged;
+  double rel_residual;  /* final true FP64 ||b-Ax||/||b||                            */
+} pa_solver_stats;
+
+/* applyBlockConstraints<3> (src/ApplyConstraints.hpp:11-79): BC rows/cols of `values`
+ * zeroed with unit diagonal, rhs lifted and set to scale*value. */
+int pa_apply_constraints(pa_handle h, double* values, double* rhs, double scale);
+
+/* AbstractSolver::solve(A, x, b) (src/alg/PlatoAbstractSolver.hpp:19-27) for a
+ * constrained block-CRS matrix on the handle's graph: Jacobi(3x3 block)-PCG in FP64. */
+int pa_solve(pa_handle h, const double* values, double* x, const double* b,
+             const pa_solver_opts* opts, pa_solver_stats* stats);
+
+/* Problem::solution (src/elliptic/Problem.hpp:259-311): u = 0, rhs = -R(0,z) = f,
+ * K = gradient_u, constraints, solve, u += du.  `values` is caller scratch of 9*nnz_b
+ * doubles (may be NULL for precision 1, which never forms K). */
+int pa_solution(pa_handle h, const double* z, double* u, double* values,
+                const pa_solver_opts* opts, pa_solver_stats* stats);
+
+/* Problem::criterionGradient for the self-adjoint compliance
+ * (src/elliptic/Problem.hpp:413-468, :437-441): dC/dz = -partial dC/dz at the solved u.
+ * `value` (host, may be NULL) receives C. */
+int pa_compliance_gradient(pa_handle h, const double* u, const double* z, double* dcdz,
+                           double* value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PLATO_MI355X_H */

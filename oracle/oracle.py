@@ -1,0 +1,148 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's
+cpu_baseline).  ctypes wrapper of oracle/_build/liboracle.so, the CPU restatement of
+the reference's Tet4 elastostatics path (see oracle.cpp for the function-by-function
+citations and what pins each function).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            raise ImportError(f"{LIB} not built (python -m platoanalyze_amd.build)")
+        _lib = C.CDLL(str(LIB))
+        _lib.orc_stress_pnorm.restype = C.c_double
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def set_threads(n: int) -> None:
+    lib().orc_set_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def graph(conn, nn):
+    conn = _i32(conn)
+    ne = conn.shape[0]
+    rm = np.empty(nn + 1, np.int32)
+    nnz = C.c_int()
+    lib().orc_graph(nn, ne, _p(conn), _p(rm), None, C.byref(nnz))
+    ci = np.empty(nnz.value, np.int32)
+    lib().orc_graph(nn, ne, _p(conn), _p(rm), _p(ci), C.byref(nnz))
+    return rm, ci
+
+
+class Model:
+    """Mesh + material + penalty (+ traction faces) for the oracle calls."""
+
+    def __init__(self, coords, conn, E=1.0, nu=0.3, p=3.0, rmin=1e-9, face_verts=None, traction=None):
+        self.X = _f64(coords).reshape(-1, 3)
+        self.conn = _i32(conn).reshape(-1, 4)
+        self.nn, self.ne = self.X.shape[0], self.conn.shape[0]
+        self.E, self.nu, self.p, self.rmin = float(E), float(nu), float(p), float(rmin)
+        self.fv = None if face_verts is None else _i32(face_verts).reshape(-1, 3)
+        self.t = None if traction is None else _f64(traction)
+
+    def _m(self):
+        return (self.nn, self.ne, _p(self.X), _p(self.conn))
+
+    def _mat(self):
+        return (C.c_double(self.E), C.c_double(self.nu), C.c_double(self.p), C.c_double(self.rmin))
+
+    def residual(self, u, z):
+        R = np.empty(3 * self.nn)
+        nf = 0 if self.fv is None else self.fv.shape[0]
+        lib().orc_residual(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), nf, _p(self.fv), _p(self.t), _p(R))
+        return R
+
+    def jacobian(self, u, z, rowmap, colidx):
+        v = np.empty(9 * int(rowmap[-1]))
+        lib().orc_jacobian(*self._m(), _p(None if u is None else _f64(u)), _p(_f64(z)), *self._mat(),
+                           _p(_i32(rowmap)), _p(_i32(colidx)), _p(v))
+        return v
+
+    def gradient_z(self, u, z, rowmap, colidx):
+        v = np.empty(3 * int(rowmap[-1]))
+        lib().orc_gradient_z(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), _p(_i32(rowmap)),
+                             _p(_i32(colidx)), _p(v))
+        return v
+
+    def energy(self, u, z, want_grad_u=True, want_grad_z=True):
+        val = C.c_double()
+        gu = np.empty(3 * self.nn) if want_grad_u else None
+        gz = np.empty(self.nn) if want_grad_z else None
+        lib().orc_energy(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), C.byref(val), _p(gu), _p(gz))
+        return val.value, gu, gz
+
+    def cell_worksets(self, u, ncells):
+        g = np.empty((ncells, 4, 3))
+        vol = np.empty(ncells)
+        eps = np.empty((ncells, 6))
+        sig = np.empty((ncells, 6))
+        res = np.empty((ncells, 12))
+        lib().orc_cell_worksets(*self._m(), _p(_f64(u)), C.c_double(self.E), C.c_double(self.nu), ncells,
+                                _p(g), _p(vol), _p(eps), _p(sig), _p(res))
+        return g, vol, eps, sig, res
+
+    def stress_pnorm(self, u, z, pnorm):
+        return float(lib().orc_stress_pnorm(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(),
+                                            C.c_double(pnorm)))
+
+
+def apply_constraints(rowmap, colidx, values, rhs, dofs, bcvals, scale=1.0):
+    values = _f64(values).copy()
+    rhs = _f64(rhs).copy()
+    dofs = _i32(dofs)
+    bcvals = _f64(bcvals)
+    lib().orc_apply_constraints(len(rowmap) - 1, _p(_i32(rowmap)), _p(_i32(colidx)), _p(values), _p(rhs),
+                                len(dofs), _p(dofs), _p(bcvals), C.c_double(scale))
+    return values, rhs
+
+
+def matvec(rowmap, colidx, values, x):
+    y = np.empty(3 * (len(rowmap) - 1))
+    lib().orc_matvec(len(rowmap) - 1, _p(_i32(rowmap)), _p(_i32(colidx)), _p(_f64(values)), _p(_f64(x)), _p(y))
+    return y
+
+
+def pcg(rowmap, colidx, values, b, tol=1e-10, maxit=100000):
+    x = np.empty(3 * (len(rowmap) - 1))
+    rel = C.c_double()
+    it = lib().orc_pcg(len(rowmap) - 1, _p(_i32(rowmap)), _p(_i32(colidx)), _p(_f64(values)), _p(_f64(b)),
+                       _p(x), C.c_double(tol), int(maxit), C.byref(rel))
+    return x, int(it), rel.value
+
+
+def to_dense(rowmap, colidx, values, bs=3):
+    """Dense matrix of a bs x bs block CRS (PlatoTestHelpers toFull analogue)."""
+    n = len(rowmap) - 1
+    A = np.zeros((bs * n, bs * n))
+    v = np.asarray(values).reshape(-1, bs, bs)
+    for r in range(n):
+        for s in range(rowmap[r], rowmap[r + 1]):
+            c = colidx[s]
+            A[bs * r:bs * r + bs, bs * c:bs * c + bs] = v[s]
+    return A

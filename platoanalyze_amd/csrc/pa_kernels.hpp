@@ -1,0 +1,62 @@
+// Device kernels of the Tet4 SIMP elastostatics path (gfx950).
+//
+// Element math follows the reference functors:
+//   gradients/volume   ComputeGradientWorkset<3>  src/ImplicitFunctors.hpp:165-217
+//   strain (Voigt)     Strain<3>                  src/Strain.hpp:55-95
+//   stress             LinearStress<3>            src/LinearStress.hpp:60-77
+//   MSIMP weighting    ApplyWeighting + MSIMP     src/ApplyWeighting.hpp:49-65, src/Simp.hpp:130-134
+//   divergence         StressDivergence<3>        src/StressDivergence.hpp:40-89
+//   energy             ScalarProduct<6>           src/ScalarProduct.hpp:28-42
+// The Jacobian block of an isotropic element is written in closed form,
+//   K_ab = vol*w*(lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I),
+// which is what the reference's SFad<double,12> evaluation of the same residual yields.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pa {
+
+constexpr int kThreads = 256;
+constexpr int kNumXcd = 8;
+
+struct Material {
+  double c00, c01, c33;  // Voigt stiffness: c(1-nu), c*nu, c(1-2nu)/2
+  double p, rmin;        // MSIMP exponent and minimum value
+};
+
+// Arguments of the chunked element kernels (one work-group per row chunk).
+struct ChunkArgs {
+  const int4* conn;
+  const uint32_t* codes;
+  const double* X;
+  const double* z;
+  const double* u;
+  const int32_t* rowmap;
+  const int32_t* chunk_rows;
+  const int32_t* list_offs;
+  const uint32_t* list;
+  double* out_vals;   // Jacobian values (9/block) or dR/dz (3/block)
+  double* out_vec;    // residual (3/row) or nodal gradient (1/row)
+  double* partials;   // per-chunk partial sums (energy)
+  // traction load of the chunk's rows, subtracted from the residual (R = K u - f)
+  const int32_t* load_offs;  // [nchunks+1]
+  const int32_t* load_row;   // row index relative to the chunk's first row
+  const double* load_f;      // 3 per entry
+  int sub_load;
+  int nchunks;
+  int max_rows;
+  int max_blocks;
+  Material m;
+};
+
+enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
+                 kModeEnergy = 4, kModeEnergyGradZ = 5 };
+
+template <int BITS, int MODE>
+__global__ void k_chunk(ChunkArgs a);
+
+// launch helpers (defined in pa_kernels.hip)
+size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks);
+hipError_t launch_chunk(int bits, int mode, const ChunkArgs& a, hipStream_t s);
+
+}  // namespace pa

@@ -1,0 +1,251 @@
+"""Drop-in surface of the reference's elliptic managers over torch device tensors.
+
+Mirrors (same names, argument meaning and error behaviour -- failures raise):
+  Plato::Elliptic::VectorFunction         src/elliptic/VectorFunction.hpp:231-689
+  Plato::Elliptic::PhysicsScalarFunction  src/elliptic/PhysicsScalarFunction.hpp:191-495
+  Plato::Elliptic::Problem                src/elliptic/Problem.hpp:259-468
+Every compute call goes through libplato_mi355x.so; torch provides device memory and
+the stream.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _capi
+from .mesh import TetMesh
+
+
+def _dptr(t: torch.Tensor):
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _hptr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class CrsMatrix:
+    """Block CRS (src/alg/CrsMatrix.hpp:20-111): rowMap, columnIndices, entries."""
+    row_map: torch.Tensor
+    column_indices: torch.Tensor
+    entries: torch.Tensor
+    block_rows: int
+    block_cols: int
+
+    def rowMap(self):  # noqa: N802 - reference name
+        return self.row_map
+
+    def columnIndices(self):  # noqa: N802
+        return self.column_indices
+
+    def entries_(self):
+        return self.entries
+
+
+@dataclass
+class EssentialBC:
+    """One "Essential Boundary Conditions" entry: node set, dof index, value."""
+    nodes: np.ndarray
+    dof: int
+    value: float = 0.0
+
+
+@dataclass
+class Traction:
+    """One "Natural Boundary Conditions" Uniform traction on a side set."""
+    face_verts: np.ndarray
+    face_elems: np.ndarray
+    traction: tuple
+
+
+@dataclass
+class ElasticModel:
+    """Handle owner: mesh + isotropic material + MSIMP penalty + BCs + device graph."""
+    mesh: TetMesh
+    youngs_modulus: float = 1.0
+    poissons_ratio: float = 0.3
+    penalty_exponent: float = 3.0
+    penalty_minimum: float = 1e-9
+    essential_bcs: list = field(default_factory=list)
+    tractions: list = field(default_factory=list)
+    graph: tuple | None = None  # optional (rowmap, colidx) host arrays
+    device: str = "cuda"
+
+    def __post_init__(self):
+        h = C.c_void_p()
+        coords = np.ascontiguousarray(self.mesh.coords, np.float64)
+        conn = np.ascontiguousarray(self.mesh.conn, np.int32)
+        _capi.call("pa_create", C.byref(h), self.mesh.num_nodes, self.mesh.num_elems, _hptr(coords), _hptr(conn))
+        self._h = h
+        _capi.call("pa_set_material", h, float(self.youngs_modulus), float(self.poissons_ratio))
+        _capi.call("pa_set_penalty", h, float(self.penalty_exponent), float(self.penalty_minimum))
+        for bc in self.essential_bcs:
+            nodes = np.ascontiguousarray(bc.nodes, np.int32)
+            _capi.call("pa_add_essential_bc", h, len(nodes), _hptr(nodes), int(bc.dof), float(bc.value))
+        for tr in self.tractions:
+            fv = np.ascontiguousarray(tr.face_verts, np.int32)
+            fe = np.ascontiguousarray(tr.face_elems, np.int32)
+            t = np.ascontiguousarray(tr.traction, np.float64)
+            _capi.call("pa_add_traction", h, len(fe), _hptr(fv), _hptr(fe), _hptr(t))
+        if self.graph is not None:
+            rm = np.ascontiguousarray(self.graph[0], np.int32)
+            ci = np.ascontiguousarray(self.graph[1], np.int32)
+            _capi.call("pa_set_graph", h, _hptr(rm), _hptr(ci))
+        _capi.call("pa_finalize", h)
+        nr, nb = C.c_int(), C.c_int()
+        _capi.call("pa_graph_size", h, C.byref(nr), C.byref(nb))
+        self.num_rows, self.nnz_blocks = nr.value, nb.value
+        self.row_map = torch.empty(self.num_rows + 1, dtype=torch.int32, device=self.device)
+        self.column_indices = torch.empty(self.nnz_blocks, dtype=torch.int32, device=self.device)
+        self._stream()
+        _capi.call("pa_copy_graph", h, _dptr(self.row_map), _dptr(self.column_indices), 1)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_nodes(self) -> int:
+        return self.mesh.num_nodes
+
+    @property
+    def num_dofs(self) -> int:
+        return 3 * self.mesh.num_nodes
+
+    def _stream(self):
+        _capi.call("pa_set_stream", self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _capi.lib().pa_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class VectorFunction:
+    """Plato::Elliptic::VectorFunction<Mechanics<3>> for the elastostatics residual."""
+
+    def __init__(self, model: ElasticModel):
+        self.m = model
+
+    def _check(self, u, z):
+        if u is not None and (u.dtype != torch.float64 or u.numel() != self.m.num_dofs):
+            raise ValueError("state must be float64 with 3*num_nodes entries")
+        if z.dtype != torch.float64 or z.numel() != self.m.num_nodes:
+            raise ValueError("control must be float64 with num_nodes entries")
+
+    def value(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
+        self._check(u, z)
+        R = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        self.m._stream()
+        _capi.call("pa_vf_value", self.m.handle, _dptr(u), _dptr(z), _dptr(R))
+        return R
+
+    def gradient_u(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0,
+                   residual_out: torch.Tensor | None = None) -> CrsMatrix:
+        self._check(u, z)
+        vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+        self.m._stream()
+        _capi.call("pa_vf_gradient_u", self.m.handle, _dptr(u) if u is not None else None, _dptr(z), _dptr(vals),
+                   _dptr(residual_out) if residual_out is not None else None)
+        return CrsMatrix(self.m.row_map, self.m.column_indices, vals, 3, 3)
+
+    gradient_u_T = gradient_u  # the elastostatic Jacobian is symmetric
+
+    def gradient_z(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> CrsMatrix:
+        self._check(u, z)
+        vals = torch.empty(3 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+        self.m._stream()
+        _capi.call("pa_vf_gradient_z", self.m.handle, _dptr(u), _dptr(z), _dptr(vals))
+        return CrsMatrix(self.m.row_map, self.m.column_indices, vals, 1, 3)
+
+
+class PhysicsScalarFunction:
+    """Plato::Elliptic::PhysicsScalarFunction for "Internal Elastic Energy"."""
+
+    def __init__(self, model: ElasticModel, function_type: str = "Internal Elastic Energy"):
+        if function_type != "Internal Elastic Energy":
+            raise ValueError(f"unsupported scalar function type {function_type!r}")
+        self.m = model
+
+    def value(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> float:
+        v = C.c_double()
+        self.m._stream()
+        _capi.call("pa_sf_value", self.m.handle, _dptr(u), _dptr(z), C.byref(v))
+        return v.value
+
+    def gradient_u(self, u: torch.Tensor, z: torch.Tensor, step: int = 0, time_step: float = 0.0) -> torch.Tensor:
+        g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        self.m._stream()
+        _capi.call("pa_sf_gradient_u", self.m.handle, _dptr(u), _dptr(z), _dptr(g))
+        return g
+
+    def gradient_z(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
+        g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=u.device)
+        self.m._stream()
+        _capi.call("pa_sf_gradient_z", self.m.handle, _dptr(u), _dptr(z), _dptr(g), None)
+        return g
+
+
+class Problem:
+    """Plato::Elliptic::Problem (self-adjoint compliance)."""
+
+    def __init__(self, model: ElasticModel, tolerance: float = 1e-10, max_iters: int = 200000,
+                 check_every: int = 16):
+        self.m = model
+        self.opts = _capi.SolverOpts(tolerance, max_iters, check_every, 0, 0, 1e-4)
+        self.stats = _capi.SolverStats()
+        self._vals = None
+        self.state = None
+
+    def solution(self, z: torch.Tensor) -> torch.Tensor:
+        u = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device)
+        if self._vals is None:
+            self._vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+        self.m._stream()
+        _capi.call("pa_solution", self.m.handle, _dptr(z), _dptr(u), _dptr(self._vals), C.byref(self.opts),
+                   C.byref(self.stats))
+        self.state = u
+        return u
+
+    def criterionValue(self, z: torch.Tensor) -> float:  # noqa: N802
+        u = self.solution(z)
+        return PhysicsScalarFunction(self.m).value(u, z)
+
+    def criterionGradient(self, z: torch.Tensor, u: torch.Tensor | None = None):  # noqa: N802
+        if u is None:
+            u = self.solution(z)
+        g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=z.device)
+        v = C.c_double()
+        self.m._stream()
+        _capi.call("pa_compliance_gradient", self.m.handle, _dptr(u), _dptr(z), _dptr(g), C.byref(v))
+        return g, v.value
+
+
+def solve(model: ElasticModel, values: torch.Tensor, b: torch.Tensor, tolerance: float = 1e-10,
+          max_iters: int = 200000, x0: torch.Tensor | None = None):
+    """AbstractSolver::solve(A, x, b) on a constrained matrix of the model's graph."""
+    x = x0.clone() if x0 is not None else torch.empty_like(b)
+    opts = _capi.SolverOpts(tolerance, max_iters, 16, 0, 1 if x0 is not None else 0, 1e-4)
+    st = _capi.SolverStats()
+    model._stream()
+    _capi.call("pa_solve", model.handle, _dptr(values), _dptr(x), _dptr(b), C.byref(opts), C.byref(st))
+    return x, st
+
+
+def apply_constraints(model: ElasticModel, values: torch.Tensor, rhs: torch.Tensor, scale: float = 1.0):
+    model._stream()
+    _capi.call("pa_apply_constraints", model.handle, _dptr(values), _dptr(rhs), float(scale))
