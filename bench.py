@@ -66,6 +66,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB device copy (for scaling HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the assembly/solve path runs on a single MI355X "
@@ -80,6 +82,15 @@ def main() -> int:
     from platoanalyze_amd.elliptic import Problem
 
     dev = "cuda"
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device=dev)  # 1 GiB
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup + args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
+                          "launches": args.warmup + args.steps}))
+        return 0
     t0 = time.perf_counter()
     w = wl.make(args.config)
     t_mesh = time.perf_counter() - t0

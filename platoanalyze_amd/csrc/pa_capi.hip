@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -29,8 +30,9 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 // LDS budget of one assembly work-group (two work-groups per CU fit in 160 KiB).
-constexpr int kMaxChunkBlocks = 960;
+constexpr int kMaxChunkBlocks = 896;
 constexpr int kMaxChunkRows = 128;
+constexpr int kMaxChunkNodes = 256;  // node table entries (rows + coupled nodes) per chunk
 constexpr int kRedGrid = 1024;  // grid of the reduction kernels
 
 template <typename T>
@@ -81,6 +83,9 @@ struct pa_problem {
   DevArray<int32_t> rowmap, colidx, diag_slot;
   DevArray<int32_t> chunk_rows, list_offs;
   DevArray<uint32_t> list;
+  DevArray<int32_t> chunk_node_offs, chunk_nodes;
+  DevArray<uint16_t> ent_ln;
+  DevArray<uint32_t> ent_code;
   DevArray<int32_t> load_nodes, load_offs, load_row;
   DevArray<double> load_f;
   int nload = 0;
@@ -102,6 +107,7 @@ struct pa_problem {
     m.c33 = 1.0 / 2.0 * c * (1.0 - 2.0 * nu);
     m.p = p;
     m.rmin = rmin;
+    m.pint = (p >= 1.0 && p <= 8.0 && p == std::floor(p)) ? (int)p : 0;
     return m;
   }
   pa::ChunkArgs chunk_args() const {
@@ -110,9 +116,15 @@ struct pa_problem {
     a.codes = codes.p;
     a.X = X.p;
     a.rowmap = rowmap.p;
+    a.colidx = colidx.p;
     a.chunk_rows = chunk_rows.p;
     a.list_offs = list_offs.p;
     a.list = list.p;
+    a.node_offs = chunk_node_offs.p;
+    a.nodes = chunk_nodes.p;
+    a.ent_ln = ent_ln.p;
+    a.ent_code = ent_code.p;
+    a.max_nodes = chunks.max_nodes;
     a.partials = chunk_partials.p;
     a.load_offs = load_offs.p;
     a.load_row = load_row.p;
@@ -297,8 +309,11 @@ int pa_finalize(pa_handle h) {
   std::vector<uint32_t> codes = pa::slot_codes(g, h->conn.data(), h->ne, h->code_bits, err);
   if (!err.empty()) return fail(PA_ERR_MESH, err);
   pa::NodeElems adj = pa::node_elems(h->conn.data(), h->ne, h->nn);
-  h->chunks = pa::build_chunks(g, adj, h->conn.data(), kMaxChunkBlocks, kMaxChunkRows);
-  if (h->chunks.max_blocks > kMaxChunkBlocks)
+  int max_blocks = kMaxChunkBlocks;
+  if (const char* env = std::getenv("PA_CHUNK_BLOCKS")) max_blocks = std::max(64, std::min(4096, std::atoi(env)));
+  const int wpc = h->code_bits == 4 ? 2 : 4;
+  h->chunks = pa::build_chunks(g, adj, h->conn.data(), codes, wpc, max_blocks, kMaxChunkRows, kMaxChunkNodes);
+  if (h->chunks.max_blocks > max_blocks || h->chunks.max_nodes > kMaxChunkNodes)
     return fail(PA_ERR_UNSUPPORTED, "a single graph row exceeds the LDS chunk budget");
   std::vector<int32_t> diag(h->nn);
   for (int r = 0; r < h->nn; ++r) {
@@ -331,6 +346,10 @@ int pa_finalize(pa_handle h) {
   PA_HIP(h->chunk_rows.upload(h->chunks.row_begin));
   PA_HIP(h->list_offs.upload(h->chunks.list_offs));
   PA_HIP(h->list.upload(h->chunks.list));
+  PA_HIP(h->chunk_node_offs.upload(h->chunks.node_offs));
+  PA_HIP(h->chunk_nodes.upload(h->chunks.nodes));
+  PA_HIP(h->ent_ln.upload(h->chunks.ent_ln));
+  PA_HIP(h->ent_code.upload(h->chunks.ent_code));
   PA_HIP(h->load_nodes.upload(ln));
   PA_HIP(h->load_offs.upload(loffs));
   PA_HIP(h->load_row.upload(lrow));

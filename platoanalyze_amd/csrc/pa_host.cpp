@@ -5,6 +5,7 @@
 #include <functional>
 #include <numeric>
 #include <thread>
+#include <unordered_map>
 
 #include "pa_host.hpp"
 
@@ -303,26 +304,61 @@ std::vector<uint32_t> slot_codes(const Graph& g, const int32_t* conn, int ne, in
   return codes;
 }
 
-Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, int max_blocks, int max_rows) {
+Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, const std::vector<uint32_t>& codes,
+                    int wpc, int max_blocks, int max_rows, int max_nodes) {
   Chunks ch;
   const int nn = g.num_rows;
-  ch.row_begin.push_back(0);
-  int r = 0;
-  while (r < nn) {
-    int r0 = r;
-    int b0 = g.rowmap[r0];
-    while (r < nn && (r - r0) < max_rows && (g.rowmap[r + 1] - b0) <= max_blocks) ++r;
-    if (r == r0) ++r;  // a single row larger than the budget still forms a chunk (caught by caller)
-    ch.max_rows = std::max(ch.max_rows, r - r0);
-    ch.max_blocks = std::max(ch.max_blocks, g.rowmap[r] - b0);
-    ch.row_begin.push_back(r);
+  // greedy partition: add rows while blocks, rows and node-table size fit
+  {
+    std::vector<int32_t> mark(nn, -1);
+    ch.row_begin.push_back(0);
+    int r = 0, cid = 0;
+    while (r < nn) {
+      const int r0 = r, b0 = g.rowmap[r0];
+      int nodes = 0;
+      while (r < nn && (r - r0) < max_rows && (g.rowmap[r + 1] - b0) <= max_blocks) {
+        int add = 0;
+        for (int k = g.rowmap[r]; k < g.rowmap[r + 1]; ++k)
+          if (mark[g.colidx[k]] != cid) ++add;
+        if (nodes + add > max_nodes && r > r0) break;
+        for (int k = g.rowmap[r]; k < g.rowmap[r + 1]; ++k)
+          if (mark[g.colidx[k]] != cid) mark[g.colidx[k]] = cid;
+        nodes += add;
+        ++r;
+      }
+      if (r == r0) ++r;  // one row over budget still forms a chunk; the caller rejects it
+      ch.row_begin.push_back(r);
+      ++cid;
+    }
   }
   ch.count = (int)ch.row_begin.size() - 1;
   std::vector<std::vector<uint32_t>> lists(ch.count);
+  std::vector<std::vector<int32_t>> tables(ch.count);
+  std::vector<std::vector<uint16_t>> lns(ch.count);
+  std::vector<int> mrows(ch.count), mblocks(ch.count), mnodes(ch.count);
   parallel_for(ch.count, [&](int64_t b, int64_t e) {
     std::vector<uint64_t> tmp;
+    std::vector<int32_t> local;  // global -> local, lazily sized
+    std::unordered_map<int32_t, int32_t> lmap;
     for (int64_t c = b; c < e; ++c) {
-      int r0 = ch.row_begin[c], r1 = ch.row_begin[c + 1];
+      const int r0 = ch.row_begin[c], r1 = ch.row_begin[c + 1];
+      // node table: rows first, then the other coupled nodes ascending
+      auto& T = tables[c];
+      T.clear();
+      for (int rr = r0; rr < r1; ++rr) T.push_back(rr);
+      std::vector<int32_t> halo;
+      for (int rr = r0; rr < r1; ++rr)
+        for (int k = g.rowmap[rr]; k < g.rowmap[rr + 1]; ++k)
+          if (g.colidx[k] < r0 || g.colidx[k] >= r1) halo.push_back(g.colidx[k]);
+      std::sort(halo.begin(), halo.end());
+      halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+      T.insert(T.end(), halo.begin(), halo.end());
+      lmap.clear();
+      for (size_t i = 0; i < T.size(); ++i) lmap[T[i]] = (int32_t)i;
+      mrows[c] = r1 - r0;
+      mblocks[c] = g.rowmap[r1] - g.rowmap[r0];
+      mnodes[c] = (int)T.size();
+      // entries
       tmp.clear();
       for (int rr = r0; rr < r1; ++rr)
         for (int k = adj.offs[rr]; k < adj.offs[rr + 1]; ++k) {
@@ -339,16 +375,60 @@ Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, i
         while (i < tmp.size() && (tmp[i] >> 4) == el) mask |= (uint32_t)(tmp[i++] & 15);
         L.push_back((uint32_t)el | (mask << 28));
       }
+      // most-owned first, element order within a group: lanes of a wavefront then
+      // loop over the same number of owned vertices
+      std::stable_sort(L.begin(), L.end(), [](uint32_t x, uint32_t y) {
+        return __builtin_popcount(x >> 28) > __builtin_popcount(y >> 28);
+      });
+      // Within each group, interleave with stride ceil(n/64): the 64 lanes of a
+      // wavefront then take elements spread over the chunk instead of neighbours
+      // sharing vertices, so their LDS atomic adds rarely hit the same address.
+      std::vector<uint32_t> tmpL(L.size());
+      size_t g0 = 0;
+      while (g0 < L.size()) {
+        size_t g1 = g0;
+        const int pc = __builtin_popcount(L[g0] >> 28);
+        while (g1 < L.size() && __builtin_popcount(L[g1] >> 28) == pc) ++g1;
+        const size_t n = g1 - g0, q = (n + 63) / 64;
+        size_t k = 0;
+        for (size_t col = 0; col < q; ++col)
+          for (size_t row = col; row < n; row += q) tmpL[g0 + k++] = L[g0 + row];
+        g0 = g1;
+      }
+      L.swap(tmpL);
+      auto& LN = lns[c];
+      LN.resize(4 * L.size());
+      for (size_t i = 0; i < L.size(); ++i) {
+        const int el = (int)(L[i] & 0x0FFFFFFFu);
+        for (int a = 0; a < 4; ++a) LN[4 * i + a] = (uint16_t)lmap.at(conn[4 * el + a]);
+      }
     }
   });
-  ch.list_offs.resize(ch.count + 1);
-  ch.list_offs[0] = 0;
-  for (int c = 0; c < ch.count; ++c) ch.list_offs[c + 1] = ch.list_offs[c] + (int32_t)lists[c].size();
+  ch.list_offs.assign(ch.count + 1, 0);
+  ch.node_offs.assign(ch.count + 1, 0);
+  for (int c = 0; c < ch.count; ++c) {
+    ch.list_offs[c + 1] = ch.list_offs[c] + (int32_t)lists[c].size();
+    ch.node_offs[c + 1] = ch.node_offs[c] + (int32_t)tables[c].size();
+    ch.max_rows = std::max(ch.max_rows, mrows[c]);
+    ch.max_blocks = std::max(ch.max_blocks, mblocks[c]);
+    ch.max_nodes = std::max(ch.max_nodes, mnodes[c]);
+  }
   ch.list.resize(ch.list_offs[ch.count]);
+  ch.nodes.resize(ch.node_offs[ch.count]);
+  ch.ent_ln.resize(4 * (size_t)ch.list_offs[ch.count]);
+  ch.ent_code.resize((size_t)wpc * ch.list_offs[ch.count]);
   parallel_for(ch.count, [&](int64_t b, int64_t e) {
-    for (int64_t c = b; c < e; ++c) std::copy(lists[c].begin(), lists[c].end(), ch.list.begin() + ch.list_offs[c]);
+    for (int64_t c = b; c < e; ++c) {
+      std::copy(lists[c].begin(), lists[c].end(), ch.list.begin() + ch.list_offs[c]);
+      std::copy(tables[c].begin(), tables[c].end(), ch.nodes.begin() + ch.node_offs[c]);
+      std::copy(lns[c].begin(), lns[c].end(), ch.ent_ln.begin() + 4 * (size_t)ch.list_offs[c]);
+      for (size_t i = 0; i < lists[c].size(); ++i) {
+        const size_t el = lists[c][i] & 0x0FFFFFFFu;
+        for (int wd = 0; wd < wpc; ++wd)
+          ch.ent_code[(size_t)wpc * (ch.list_offs[c] + i) + wd] = codes[(size_t)wpc * el + wd];
+      }
+    }
   });
-  (void)conn;
   return ch;
 }
 

@@ -59,18 +59,28 @@ NodeElems node_elems(const int32_t* conn, int ne, int nn);
 // `bits` bits per (a,b) pair (index 4a+b). bits = 4 (rows <= 16) or 8 (rows <= 256).
 std::vector<uint32_t> slot_codes(const Graph& g, const int32_t* conn, int ne, int bits, std::string& err);
 
-// Row chunks for the LDS assembly: consecutive rows whose block count fits the
-// LDS budget, plus the list of elements touching each chunk. List entries hold the
-// element id in the low 28 bits and the mask of locally-owned vertices in the top 4.
+// Row chunks for the LDS assembly: consecutive rows whose blocks, rows and node
+// table fit the LDS budget.  Per chunk:
+//   * node table: the chunk's rows first (local index = row - first row), then the
+//     other nodes its rows couple to (the union of the rows' graph columns), sorted;
+//   * entries: every element touching one of the chunk's rows, as
+//       list     element id (low 28 bits) | mask of owned vertices (top 4 bits),
+//       ent_ln   the element's 4 vertices as local node-table indices (uint16),
+//       ent_code the element's slot codes (words_per_code uint32 each),
+//     sorted by owned-vertex count and interleaved (see build_chunks).
 struct Chunks {
   int count = 0;
   std::vector<int32_t> row_begin;    // [count+1]
   std::vector<int32_t> list_offs;    // [count+1]
   std::vector<uint32_t> list;        // element | (mask << 28)
-  int max_rows = 0, max_blocks = 0;
+  std::vector<int32_t> node_offs;    // [count+1]
+  std::vector<int32_t> nodes;        // node tables
+  std::vector<uint16_t> ent_ln;      // 4 per entry
+  std::vector<uint32_t> ent_code;    // words_per_code per entry
+  int max_rows = 0, max_blocks = 0, max_nodes = 0;
 };
-Chunks build_chunks(const Graph& g, const NodeElems& ne_adj, const int32_t* conn, int max_blocks,
-                    int max_rows);
+Chunks build_chunks(const Graph& g, const NodeElems& ne_adj, const int32_t* conn, const std::vector<uint32_t>& codes,
+                    int words_per_code, int max_blocks, int max_rows, int max_nodes);
 
 // Minimal thread pool helper for setup loops.
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& body);

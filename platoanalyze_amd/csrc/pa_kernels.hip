@@ -19,15 +19,6 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int BITS>
-__device__ __forceinline__ int slot_of(const uint32_t* cw, int a, int b) {
-  if constexpr (BITS == 4) {
-    return (cw[a >> 1] >> (16 * (a & 1) + 4 * b)) & 15u;
-  } else {
-    return (cw[a] >> (8 * b)) & 255u;
-  }
-}
-
 // ComputeGradientWorkset<3>: J[d][k] = X_k[d] - X_3[d]; grad N_k = row k of J^-1
 // (k<3), grad N_3 = -(sum); vol = |det J| * 1/6 (LinearTetCubRuleDegreeOne weight).
 __device__ __forceinline__ void tet_gradients(const double x[4][3], double g[4][3], double& vol) {
@@ -84,8 +75,29 @@ __device__ __forceinline__ void tet_divergence(const double s[6], const double g
   r[2] = vol * (s[4] * ga[0] + s[3] * ga[1] + s[2] * ga[2]);
 }
 
-template <int BITS, int MODE>
-__global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
+// select x[a] for a runtime a in 0..3 without dynamic register indexing
+__device__ __forceinline__ double sel4(int a, double x0, double x1, double x2, double x3) {
+  return a == 0 ? x0 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
+}
+__device__ __forceinline__ int sel4i(int a, int x0, int x1, int x2, int x3) {
+  return a == 0 ? x0 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
+}
+
+// MSIMP weight pow(rho, p): small integer exponents take a multiply chain (uniform
+// branch on a kernel argument); anything else uses pow().
+__device__ __forceinline__ double simp_pow(double rho, double p, int pint) {
+  if (pint > 0) {
+    double r = rho;
+    for (int k = 1; k < pint; ++k) r *= rho;
+    return r;
+  }
+  return pow(rho, p);
+}
+
+// 512-thread groups: ask for <=128 VGPRs so two groups (4 waves per SIMD) fit a CU.
+template <int BITS, int MODE, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
+void k_chunk(ChunkArgs A) {
   extern __shared__ double lds[];
   constexpr bool kJac = MODE == kModeJacobian || MODE == kModeJacobianResidual;
   constexpr bool kRes = MODE == kModeJacobianResidual || MODE == kModeResidual;
@@ -106,16 +118,18 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
   int* roff = reinterpret_cast<int*>(res + (size_t)A.max_rows * kRowW);
 
   if constexpr (kBlkW > 0)
-    for (int k = threadIdx.x; k < nb * kBlkW; k += kThreads) seg[k] = 0.0;
+    for (int k = threadIdx.x; k < nb * kBlkW; k += NT) seg[k] = 0.0;
   if constexpr (kRowW > 0)
-    for (int k = threadIdx.x; k < nr * kRowW; k += kThreads) res[k] = 0.0;
+    for (int k = threadIdx.x; k < nr * kRowW; k += NT) res[k] = 0.0;
   if constexpr (kBlkW > 0)
-    for (int k = threadIdx.x; k <= nr; k += kThreads) roff[k] = A.rowmap[r0 + k] - b0;
+    for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
   __syncthreads();
 
   double energy = 0.0;
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
-  for (int k = l0 + threadIdx.x; k < l1; k += kThreads) {
+  // Lists are sorted by owned-vertex count, so the per-lane loops over owned
+  // vertices below have (nearly) the same trip count across a wavefront.
+  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
     const uint32_t ent = A.list[k];
     const int e = (int)(ent & 0x0FFFFFFFu);
     const uint32_t mask = ent >> 28;
@@ -132,8 +146,7 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
 #pragma unroll
     for (int a = 0; a < 4; ++a) rho += A.z[nd[a]];
     rho = rho / 4.0;
-    const double pw = pow(rho, A.m.p);
-    const double w = A.m.rmin + (1.0 - A.m.rmin) * pw;
+    const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
 
     uint32_t cw[4];
     if constexpr (kBlkW > 0) {
@@ -141,6 +154,8 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
         const uint2 t = reinterpret_cast<const uint2*>(A.codes)[e];
         cw[0] = t.x;
         cw[1] = t.y;
+        cw[2] = t.y;
+        cw[3] = t.y;
       } else {
         const uint4 t = reinterpret_cast<const uint4*>(A.codes)[e];
         cw[0] = t.x;
@@ -161,6 +176,10 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
       iso_stress(A.m, eps, s);
     }
 
+    double dw = 0.0;
+    if constexpr (kGradZ || kGzT)
+      dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
+
     if constexpr (kEnergy) {
       // ScalarProduct<6>(sigma_w, eps) * 0.5 * vol ; counted once, by the chunk owning vertex 0.
       double se = 0.0;
@@ -172,70 +191,79 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
         double sr = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) sr += s[q] * eps[q];
-        const double dw = (1.0 - A.m.rmin) * A.m.p * pow(rho, A.m.p - 1.0);
         const double gz = 0.5 * vol * sr * dw * 0.25;
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-          if (mask & (1u << a)) lds_add(&res[nd[a] - r0], gz);
+        for (uint32_t bits = mask; bits; bits &= bits - 1) {
+          const int a = __builtin_ctz(bits);
+          lds_add(&res[sel4i(a, nd[0], nd[1], nd[2], nd[3]) - r0], gz);
+        }
       }
     }
 
+    double sw[6];
     if constexpr (kRes) {
-      double sw[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) sw[q] = w * s[q];
+    }
+    // closed-form block K_ab = L_a g_b^T + M_b g_a^T + (M_b.g_a) I with L = vw*lam*g,
+    // M = vw*mu*g (vw = vol * w)
+    const double vw = vol * w;
+    double rb[4][3];
+    if constexpr (kGzT) {
+      // dR_b/dz_a = dw/drho * 1/4 * (vol B_b^T sigma), rows = control node a.
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (mask & (1u << a)) {
+      for (int b = 0; b < 4; ++b) {
+        tet_divergence(s, g[b], vol, rb[b]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rb[b][i] *= dw * 0.25;
+      }
+    }
+
+    if constexpr (kJac || kRes || kGzT) {
+      for (uint32_t bits = mask; bits; bits &= bits - 1) {
+        const int a = __builtin_ctz(bits);
+        const int lr = sel4i(a, nd[0], nd[1], nd[2], nd[3]) - r0;
+        double ga[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ga[i] = sel4(a, g[0][i], g[1][i], g[2][i], g[3][i]);
+        if constexpr (kRes) {
           double r[3];
-          tet_divergence(sw, g[a], vol, r);
-          double* dst = &res[3 * (nd[a] - r0)];
+          tet_divergence(sw, ga, vol, r);
+          double* dst = &res[3 * lr];
           lds_add(dst + 0, r[0]);
           lds_add(dst + 1, r[1]);
           lds_add(dst + 2, r[2]);
         }
-      }
-    }
-
-    if constexpr (kJac) {
-      const double lam = A.m.c01, mu = A.m.c33;
-      const double vw = vol * w;
+        if constexpr (kJac || kGzT) {
+          const int base = roff[lr];
+          uint32_t code;
+          if constexpr (BITS == 4) code = (a < 2 ? cw[0] : cw[1]) >> (16 * (a & 1));
+          else code = a == 0 ? cw[0] : (a == 1 ? cw[1] : (a == 2 ? cw[2] : cw[3]));
+          if constexpr (kJac) {
+            const double vl = vw * A.m.c01, vm = vw * A.m.c33;
+            const double La[3] = {vl * ga[0], vl * ga[1], vl * ga[2]};
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (!(mask & (1u << a))) continue;
-        const int base = roff[nd[a] - r0];
+            for (int b = 0; b < 4; ++b) {
+              const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
+              double* dst = &seg[9 * sl];
+              const double Mb[3] = {vm * g[b][0], vm * g[b][1], vm * g[b][2]};
+              const double d = Mb[0] * ga[0] + Mb[1] * ga[1] + Mb[2] * ga[2];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int sl = base + slot_of<BITS>(cw, a, b);
-          double* dst = &seg[9 * sl];
-          const double dot = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+              for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-              double v = lam * g[a][i] * g[b][j] + mu * g[a][j] * g[b][i];
-              if (i == j) v += mu * dot;
-              lds_add(dst + 3 * i + j, vw * v);
+                for (int j = 0; j < 3; ++j) {
+                  double v = La[i] * g[b][j] + Mb[i] * ga[j];
+                  if (i == j) v += d;
+                  lds_add(dst + 3 * i + j, v);
+                }
             }
-        }
-      }
-    }
-
-    if constexpr (kGzT) {
-      // dR_b/dz_a = dw/drho * 1/4 * (vol B_b^T sigma), rows = control node a.
-      const double dw = (1.0 - A.m.rmin) * A.m.p * pow(rho, A.m.p - 1.0) * 0.25;
-      double rb[4][3];
+          } else {
 #pragma unroll
-      for (int b = 0; b < 4; ++b) tet_divergence(s, g[b], vol, rb[b]);
+            for (int b = 0; b < 4; ++b) {
+              const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (!(mask & (1u << a))) continue;
-        const int base = roff[nd[a] - r0];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int sl = base + slot_of<BITS>(cw, a, b);
-#pragma unroll
-          for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], dw * rb[b][i]);
+              for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], rb[b][i]);
+            }
+          }
         }
       }
     }
@@ -243,7 +271,7 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
 
   if constexpr (kEnergy) {
     // block reduction of the energy (fixed order -> deterministic)
-    __shared__ double wsum[kThreads / 64];
+    __shared__ double wsum[NT / 64];
     double v = energy;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -251,14 +279,14 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0.0;
-      for (int q = 0; q < kThreads / 64; ++q) t += wsum[q];
+      for (int q = 0; q < NT / 64; ++q) t += wsum[q];
       A.partials[c] = t;
     }
   }
   __syncthreads();
   if constexpr (kRes) {
     if (A.sub_load) {
-      for (int k = A.load_offs[c] + threadIdx.x; k < A.load_offs[c + 1]; k += kThreads) {
+      for (int k = A.load_offs[c] + threadIdx.x; k < A.load_offs[c + 1]; k += NT) {
         const int lr = A.load_row[k];
         res[3 * lr + 0] -= A.load_f[3 * k + 0];
         res[3 * lr + 1] -= A.load_f[3 * k + 1];
@@ -270,54 +298,287 @@ __global__ __launch_bounds__(kThreads) void k_chunk(ChunkArgs A) {
 
   if constexpr (kBlkW > 0) {
     double* dst = A.out_vals + (size_t)b0 * kBlkW;
-    for (int k = threadIdx.x; k < nb * kBlkW; k += kThreads) dst[k] = seg[k];
+    for (int k = threadIdx.x; k < nb * kBlkW; k += NT) dst[k] = seg[k];
   }
   if constexpr (kRowW > 0) {
     double* dst = A.out_vec + (size_t)r0 * kRowW;
-    for (int k = threadIdx.x; k < nr * kRowW; k += kThreads) dst[k] = res[k];
+    for (int k = threadIdx.x; k < nr * kRowW; k += NT) dst[k] = res[k];
   }
 }
 
+
+// Local edges (a,b), a<b, of a tet packed 4 bits per edge index t = 0..5.
+constexpr uint32_t kEdgeA = 0x211000u;  // 0,0,0,1,1,2
+constexpr uint32_t kEdgeB = 0x332321u;  // 1,2,3,2,3,3
+
+// LDS layout of the Jacobian kernel (byte offsets, 16-byte aligned sections).
+struct JacLds {
+  size_t seg, nodes, roff, col, fl, total;
+};
+__host__ __device__ inline JacLds jac_lds_layout(int max_rows, int max_blocks, int max_nodes) {
+  auto al = [](size_t x) { return (x + 15) & ~(size_t)15; };
+  JacLds L;
+  L.seg = 0;
+  L.nodes = al(L.seg + sizeof(double) * 9 * (size_t)max_blocks);
+  L.roff = al(L.nodes + sizeof(double) * 4 * (size_t)max_nodes);
+  L.col = al(L.roff + sizeof(int) * ((size_t)max_rows + 1));
+  L.fl = al(L.col + sizeof(int) * (size_t)max_blocks);
+  L.total = al(L.fl + sizeof(double) * 3 * (size_t)max_rows);
+  return L;
+}
+
+// Jacobian (+ residual) of one chunk of rows, accumulated per edge.
+//   * The chunk's node table (its rows, then the nodes they couple to) is staged in
+//     LDS once with near-coalesced loads; elements arrive as streamed records of
+//     four local node indices plus slot codes, so the element loop does no global
+//     gathers.  A vertex is owned by the chunk iff its local index < number of rows.
+//   * Off-diagonal blocks are accumulated once per element edge: in the row of the
+//     smaller vertex number when both ends are owned (the other block is its
+//     transpose, K being symmetric), else in the row of the owned end.
+//   * Diagonal blocks use the rigid-translation identity of every element,
+//     sum_b K_ab = 0 (sum_b grad N_b = 0): K_rr = -sum_{c != r} K_rc.
+//   * Residual R_r = sum_c K_rc u_c - f_r = sum_{c != r} K_rc (u_c - u_r) - f_r.
+// Transposes, diagonals and residual are finished by 16-lane row groups with
+// cross-lane reductions, then the chunk's CRS segment is streamed out once.
+template <int BITS, bool WITH_R, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
+void k_jac_chunk(ChunkArgs A) {
+  extern __shared__ double lds[];
+  const JacLds Lo = jac_lds_layout(A.max_rows, A.max_blocks, A.max_nodes);
+  char* base = reinterpret_cast<char*>(lds);
+  double* seg = reinterpret_cast<double*>(base + Lo.seg);
+  double4* nodes = reinterpret_cast<double4*>(base + Lo.nodes);
+  int* roff = reinterpret_cast<int*>(base + Lo.roff);
+  int* col = reinterpret_cast<int*>(base + Lo.col);
+  double* fl = reinterpret_cast<double*>(base + Lo.fl);
+
+  const int c = xcd_chunk(blockIdx.x, A.nchunks);
+  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1];
+  const int nr = r1 - r0;
+  const int b0 = A.rowmap[r0];
+  const int nb = A.rowmap[r1] - b0;
+  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+
+  for (int k = threadIdx.x; k < nb * 9; k += NT) seg[k] = 0.0;
+  for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
+  for (int k = threadIdx.x; k < nb; k += NT) col[k] = A.colidx[b0 + k];
+  for (int k = threadIdx.x; k < nnode; k += NT) {
+    const size_t gid = (size_t)A.nodes[n0 + k];
+    nodes[k] = make_double4(A.X[3 * gid], A.X[3 * gid + 1], A.X[3 * gid + 2], A.z[gid]);
+  }
+  if constexpr (WITH_R)
+    for (int k = threadIdx.x; k < nr * 3; k += NT) fl[k] = 0.0;
+  __syncthreads();
+  if constexpr (WITH_R) {
+    if (A.sub_load)
+      for (int k = A.load_offs[c] + threadIdx.x; k < A.load_offs[c + 1]; k += NT) {
+        const int lr = A.load_row[k];
+        fl[3 * lr + 0] = A.load_f[3 * k + 0];
+        fl[3 * lr + 1] = A.load_f[3 * k + 1];
+        fl[3 * lr + 2] = A.load_f[3 * k + 2];
+      }
+  }
+
+  const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
+    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+    uint32_t cw[4];
+    if constexpr (BITS == 4) {
+      const uint2 t = reinterpret_cast<const uint2*>(A.ent_code)[k];
+      cw[0] = t.x;
+      cw[1] = t.y;
+      cw[2] = t.y;
+      cw[3] = t.y;
+    } else {
+      const uint4 t = reinterpret_cast<const uint4*>(A.ent_code)[k];
+      cw[0] = t.x;
+      cw[1] = t.y;
+      cw[2] = t.z;
+      cw[3] = t.w;
+    }
+    double x[4][3], g[4][3], vol, rho = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double4 p = nodes[ln[a]];
+      x[a][0] = p.x;
+      x[a][1] = p.y;
+      x[a][2] = p.z;
+      rho += p.w;
+    }
+    tet_gradients(x, g, vol);
+    rho = rho / 4.0;
+    const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
+    const double vl = vol * w * A.m.c01, vm = vol * w * A.m.c33;
+    uint32_t mask = 0, emask = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) mask |= (ln[a] < nr ? 1u : 0u) << a;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      const int a = (kEdgeA >> (4 * t)) & 15, b = (kEdgeB >> (4 * t)) & 15;
+      emask |= (((mask >> a) | (mask >> b)) & 1u) << t;
+    }
+    // lists are sorted by owned-vertex count -> near-uniform trip counts per wave
+    for (uint32_t bits = emask; bits; bits &= bits - 1) {
+      const int t = __builtin_ctz(bits);
+      const int a = (kEdgeA >> (4 * t)) & 15, b = (kEdgeB >> (4 * t)) & 15;
+      const int la = sel4i(a, ln[0], ln[1], ln[2], ln[3]), lb = sel4i(b, ln[0], ln[1], ln[2], ln[3]);
+      const bool oa = la < nr, ob = lb < nr;
+      // both owned: the row of the smaller vertex (local order = global order for rows)
+      const bool xa_is_a = oa && (!ob || la < lb);
+      const int xa = xa_is_a ? a : b, ya = xa_is_a ? b : a;
+      const int lx = xa_is_a ? la : lb;
+      double gx[3], gy[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        gx[i] = sel4(xa, g[0][i], g[1][i], g[2][i], g[3][i]);
+        gy[i] = sel4(ya, g[0][i], g[1][i], g[2][i], g[3][i]);
+      }
+      uint32_t code;
+      if constexpr (BITS == 4) code = ((xa < 2 ? cw[0] : cw[1]) >> (16 * (xa & 1) + 4 * ya)) & 15u;
+      else code = ((xa == 0 ? cw[0] : (xa == 1 ? cw[1] : (xa == 2 ? cw[2] : cw[3]))) >> (8 * ya)) & 255u;
+      double* dst = &seg[9 * (roff[lx] + (int)code)];
+      // K_xy = vw (lam g_x g_y^T + mu g_y g_x^T + mu (g_x.g_y) I)
+      const double d = vm * (gx[0] * gy[0] + gx[1] * gy[1] + gx[2] * gy[2]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double v = vl * gx[i] * gy[j] + vm * gy[i] * gx[j];
+          if (i == j) v += d;
+          lds_add(dst + 3 * i + j, v);
+        }
+    }
+  }
+  __syncthreads();
+
+  // Row groups of 16 lanes finish one row each: transposed blocks, diagonal, residual.
+  constexpr int kG = 16;
+  const int lane = threadIdx.x % kG;
+  for (int lr = threadIdx.x / kG; lr < nr; lr += NT / kG) {
+    const int rr = r0 + lr;
+    double sum[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) sum[q] = 0.0;
+    double ru[3] = {0.0, 0.0, 0.0}, ur[3] = {0.0, 0.0, 0.0};
+    if constexpr (WITH_R) {
+      ur[0] = A.u[3 * (size_t)rr];
+      ur[1] = A.u[3 * (size_t)rr + 1];
+      ur[2] = A.u[3 * (size_t)rr + 2];
+    }
+    int dslot = -1;
+    for (int s2 = roff[lr] + lane; s2 < roff[lr + 1]; s2 += kG) {
+      const int cc = col[s2];
+      if (cc == rr) {
+        dslot = s2;
+        continue;
+      }
+      double blk[9];
+      if (cc >= r0 && cc < rr) {  // both rows in the chunk: transpose of block (cc, rr)
+        int p = -1;
+        for (int q = roff[cc - r0]; q < roff[cc - r0 + 1]; ++q)
+          if (col[q] == rr) p = q;
+        if (p >= 0) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) blk[3 * i + j] = seg[9 * p + 3 * j + i];
+#pragma unroll
+          for (int q = 0; q < 9; ++q) seg[9 * s2 + q] = blk[q];
+        } else {  // unsymmetric caller graph: keep what was accumulated
+#pragma unroll
+          for (int q = 0; q < 9; ++q) blk[q] = seg[9 * s2 + q];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) blk[q] = seg[9 * s2 + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sum[q] += blk[q];
+      if constexpr (WITH_R) {
+        const double du0 = A.u[3 * (size_t)cc] - ur[0], du1 = A.u[3 * (size_t)cc + 1] - ur[1],
+                     du2 = A.u[3 * (size_t)cc + 2] - ur[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ru[i] += blk[3 * i] * du0 + blk[3 * i + 1] * du1 + blk[3 * i + 2] * du2;
+      }
+    }
+#pragma unroll
+    for (int off = kG / 2; off > 0; off >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) sum[q] += __shfl_xor(sum[q], off, kG);
+      if constexpr (WITH_R) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ru[i] += __shfl_xor(ru[i], off, kG);
+      }
+      dslot = max(dslot, __shfl_xor(dslot, off, kG));
+    }
+    if (lane < 9 && dslot >= 0) {
+      double v = sum[0];
+#pragma unroll
+      for (int q = 1; q < 9; ++q)
+        if (lane == q) v = sum[q];
+      seg[9 * dslot + lane] = -v;
+    }
+    if constexpr (WITH_R) {
+      if (lane < 3) {
+        const double v = lane == 0 ? ru[0] : (lane == 1 ? ru[1] : ru[2]);
+        A.out_vec[3 * (size_t)rr + lane] = v - fl[3 * lr + lane];
+      }
+    }
+  }
+  __syncthreads();
+  double* dst = A.out_vals + (size_t)b0 * 9;
+  for (int k = threadIdx.x; k < nb * 9; k += NT) dst[k] = seg[k];
+}
+
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks) {
-  int blkw = (mode == kModeJacobian || mode == kModeJacobianResidual) ? 9 : (mode == kModeGradZT ? 3 : 0);
-  int roww = (mode == kModeJacobianResidual || mode == kModeResidual) ? 3 : (mode == kModeEnergyGradZ ? 1 : 0);
+  if (mode == kModeJacobian || mode == kModeJacobianResidual) return 0;  // see jac_lds_layout
+  int blkw = mode == kModeGradZT ? 3 : 0;
+  int roww = mode == kModeResidual ? 3 : (mode == kModeEnergyGradZ ? 1 : 0);
   size_t bytes = sizeof(double) * ((size_t)max_blocks * blkw + (size_t)max_rows * roww);
   if (blkw > 0) bytes += sizeof(int) * (size_t)(max_rows + 1);
   return bytes;
 }
 
+// Work-group size per mode: the LDS-heavy Jacobian modes use 512 threads so that two
+// work-groups (16 waves) share a CU's 160 KiB of LDS.
+constexpr int kNtJac = 512;
+constexpr int kNtVec = 256;
+
 template <int BITS>
 static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
-  const size_t lds = chunk_lds_bytes(mode, a.max_rows, a.max_blocks);
-  dim3 grid(a.nchunks), block(kThreads);
+  size_t lds = chunk_lds_bytes(mode, a.max_rows, a.max_blocks);
+  if (mode == kModeJacobian || mode == kModeJacobianResidual)
+    lds = jac_lds_layout(a.max_rows, a.max_blocks, a.max_nodes).total;
+  dim3 grid(a.nchunks);
   switch (mode) {
-    case kModeJacobian: k_chunk<BITS, kModeJacobian><<<grid, block, lds, s>>>(a); break;
-    case kModeJacobianResidual: k_chunk<BITS, kModeJacobianResidual><<<grid, block, lds, s>>>(a); break;
-    case kModeResidual: k_chunk<BITS, kModeResidual><<<grid, block, lds, s>>>(a); break;
-    case kModeGradZT: k_chunk<BITS, kModeGradZT><<<grid, block, lds, s>>>(a); break;
-    case kModeEnergy: k_chunk<BITS, kModeEnergy><<<grid, block, lds, s>>>(a); break;
-    case kModeEnergyGradZ: k_chunk<BITS, kModeEnergyGradZ><<<grid, block, lds, s>>>(a); break;
+    case kModeJacobian: k_jac_chunk<BITS, false, kNtJac><<<grid, kNtJac, lds, s>>>(a); break;
+    case kModeJacobianResidual: k_jac_chunk<BITS, true, kNtJac><<<grid, kNtJac, lds, s>>>(a); break;
+    case kModeResidual: k_chunk<BITS, kModeResidual, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeGradZT: k_chunk<BITS, kModeGradZT, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeEnergy: k_chunk<BITS, kModeEnergy, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeEnergyGradZ: k_chunk<BITS, kModeEnergyGradZ, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-template <int BITS, int MODE>
-static void allow_lds() {
-  hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chunk<BITS, MODE>),
-                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+template <typename K>
+static void allow_lds(K kernel) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            160 * 1024);
 }
 
 hipError_t launch_chunk(int bits, int mode, const ChunkArgs& a, hipStream_t s) {
   static bool once = false;
   if (!once) {
     once = true;
-    allow_lds<4, kModeJacobian>();
-    allow_lds<4, kModeJacobianResidual>();
-    allow_lds<4, kModeGradZT>();
-    allow_lds<8, kModeJacobian>();
-    allow_lds<8, kModeJacobianResidual>();
-    allow_lds<8, kModeGradZT>();
+    allow_lds(&k_jac_chunk<4, false, kNtJac>);
+    allow_lds(&k_jac_chunk<4, true, kNtJac>);
+    allow_lds(&k_jac_chunk<8, false, kNtJac>);
+    allow_lds(&k_jac_chunk<8, true, kNtJac>);
+    allow_lds(&k_chunk<4, kModeGradZT, kNtVec>);
+    allow_lds(&k_chunk<8, kModeGradZT, kNtVec>);
   }
   if (a.nchunks == 0) return hipSuccess;
   return bits == 4 ? launch_chunk_b<4>(mode, a, s) : launch_chunk_b<8>(mode, a, s);

@@ -22,6 +22,7 @@ constexpr int kNumXcd = 8;
 struct Material {
   double c00, c01, c33;  // Voigt stiffness: c(1-nu), c*nu, c(1-2nu)/2
   double p, rmin;        // MSIMP exponent and minimum value
+  int pint;              // p as a small positive integer (1..8), else 0
 };
 
 // Arguments of the chunked element kernels (one work-group per row chunk).
@@ -32,9 +33,14 @@ struct ChunkArgs {
   const double* z;
   const double* u;
   const int32_t* rowmap;
+  const int32_t* colidx;
   const int32_t* chunk_rows;
   const int32_t* list_offs;
   const uint32_t* list;
+  const int32_t* node_offs;   // [nchunks+1] node-table offsets
+  const int32_t* nodes;       // node tables (chunk rows first)
+  const uint16_t* ent_ln;     // 4 local node indices per entry
+  const uint32_t* ent_code;   // slot codes per entry
   double* out_vals;   // Jacobian values (9/block) or dR/dz (3/block)
   double* out_vec;    // residual (3/row) or nodal gradient (1/row)
   double* partials;   // per-chunk partial sums (energy)
@@ -46,14 +52,13 @@ struct ChunkArgs {
   int nchunks;
   int max_rows;
   int max_blocks;
+  int max_nodes;
   Material m;
 };
 
 enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
                  kModeEnergy = 4, kModeEnergyGradZ = 5 };
 
-template <int BITS, int MODE>
-__global__ void k_chunk(ChunkArgs a);
 
 // launch helpers (defined in pa_kernels.hip)
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks);
