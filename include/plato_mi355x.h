@@ -154,7 +154,11 @@ int pa_solve(pa_handle h, const double* values, double* x, const double* b,
 
 /* Problem::solution (src/elliptic/Problem.hpp:259-311): u = 0, rhs = -R(0,z) = f,
  * K = gradient_u, constraints, solve, u += du.  `values` is caller scratch of 9*nnz_b
- * doubles (may be NULL for precision 1, which never forms K). */
+ * doubles (may be NULL for precision 1, which never forms K).
+ * precision 1: FP64 iterative refinement -- true residual r = f - K u from the FP64
+ * element kernel -- around an FP32 matrix-free Jacobi-PCG that solves each correction
+ * to inner_tolerance; use_x0 = 1 starts from the caller's u (warm start; its
+ * constrained dofs are reset to their prescribed values). */
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
 

@@ -12,6 +12,7 @@
 #include "../../include/plato_mi355x.h"
 #include "pa_host.hpp"
 #include "pa_kernels.hpp"
+#include "pa_mf.hpp"
 #include "pa_solver.hpp"
 
 namespace {
@@ -91,6 +92,12 @@ struct pa_problem {
   int nload = 0;
   DevArray<int32_t> bc_d;
   DevArray<double> bc_v;
+  DevArray<uint8_t> bcmask;
+  double f_free_norm2 = 0.0;
+  // mixed-precision solver workspace
+  DevArray<float4> m_xz;
+  DevArray<float> m_dinv, m_r, m_z, m_p, m_q, m_d;
+  DevArray<double> m_R;
   DevArray<double> partials;
   DevArray<unsigned> counter;
   DevArray<double> scal;
@@ -356,6 +363,16 @@ int pa_finalize(pa_handle h) {
   PA_HIP(h->load_f.upload(lf));
   PA_HIP(h->bc_d.upload(h->bc_dofs));
   PA_HIP(h->bc_v.upload(h->bc_vals));
+  {
+    std::vector<uint8_t> bm(h->nn, 0);
+    for (int32_t d : h->bc_dofs) bm[d / 3] |= (uint8_t)(1u << (d % 3));
+    double s2 = 0.0;
+    for (int n = 0; n < h->nn; ++n)
+      for (int i = 0; i < 3; ++i)
+        if (!((bm[n] >> i) & 1u)) s2 += h->f[3 * (size_t)n + i] * h->f[3 * (size_t)n + i];
+    h->f_free_norm2 = s2;
+    PA_HIP(h->bcmask.upload(bm));
+  }
   PA_HIP(h->partials.alloc(3 * kRedGrid + 16));
   PA_HIP(h->counter.alloc(4));
   PA_HIP(hipMemset(h->counter.p, 0, 4 * sizeof(unsigned)));
@@ -557,12 +574,121 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   return PA_OK;
 }
 
+// Mixed precision: FP64 iterative refinement around an FP32 matrix-free Jacobi-PCG.
+// Outer: r = f - K u in FP64 (element residual kernel, constrained dofs masked);
+// inner: K d = r to inner_tolerance with y = K x in FP32; u += d in FP64.
+static int solve_mixed(pa_handle h, const double* z, double* u, const pa_solver_opts& opts, pa_solver_stats* st) {
+  const int nn = h->nn;
+  const size_t ndof = 3 * (size_t)nn;
+  hipStream_t s = h->stream;
+  if (h->m_r.n != ndof) {
+    PA_HIP(h->m_xz.alloc(nn));
+    PA_HIP(h->m_dinv.alloc(9 * (size_t)nn));
+    PA_HIP(h->m_r.alloc(ndof));
+    PA_HIP(h->m_z.alloc(ndof));
+    PA_HIP(h->m_p.alloc(ndof));
+    PA_HIP(h->m_q.alloc(ndof));
+    PA_HIP(h->m_d.alloc(ndof));
+    PA_HIP(h->m_R.alloc(ndof));
+  }
+  PA_HIP(pa::launch_make_xz(h->X.p, z, h->m_xz.p, nn, s));
+  pa::DiagArgs da{};
+  da.chunk_rows = h->chunk_rows.p;
+  da.node_offs = h->chunk_node_offs.p;
+  da.nodes = h->chunk_nodes.p;
+  da.ent_ln = h->ent_ln.p;
+  da.list_offs = h->list_offs.p;
+  da.nchunks = h->chunks.count;
+  da.max_rows = h->chunks.max_rows;
+  da.max_nodes = h->chunks.max_nodes;
+  da.X = h->X.p;
+  da.z = z;
+  da.bcmask = h->bcmask.p;
+  da.dinv = h->m_dinv.p;
+  da.m = h->material();
+  PA_HIP(pa::launch_mf_diag_inv(da, s));
+  if (!opts.use_x0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(u, ndof, 0.0);
+  const int nbc = (int)h->bc_dofs.size();
+  if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(u, h->bc_d.p, h->bc_v.p, nbc, 1.0);
+  PA_HIP(hipGetLastError());
+  pa::MfArgs ma{};
+  ma.chunk_rows = h->chunk_rows.p;
+  ma.node_offs = h->chunk_node_offs.p;
+  ma.nodes = h->chunk_nodes.p;
+  ma.ent_ln = h->ent_ln.p;
+  ma.list_offs = h->list_offs.p;
+  ma.nchunks = h->chunks.count;
+  ma.max_rows = h->chunks.max_rows;
+  ma.max_nodes = h->chunks.max_nodes;
+  ma.Xz = h->m_xz.p;
+  ma.bcmask = h->bcmask.p;
+  ma.x = h->m_p.p;
+  ma.y = h->m_q.p;
+  ma.partials = h->chunk_partials.p;
+  ma.counter = h->counter.p;
+  ma.scal = h->scal.p;
+  ma.scal_out = pa::kPQ;
+  ma.scal_num = pa::kRZ;
+  ma.m = h->material();
+  const double bb = h->f_free_norm2;
+  int inner_total = 0, outer = 0;
+  double rel = 0.0;
+  bool conv = false;
+  const double itol = opts.inner_tolerance > 0 ? opts.inner_tolerance : 1e-4;
+  while (true) {
+    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, true)) return rc;
+    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
+                                      h->scal.p, pa::kDot1, s));
+    if (int rc = read_scal(h)) return rc;
+    const double rr0 = h->host_scal[pa::kDot1];
+    rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
+    if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve: non-finite residual");
+    if (rel <= opts.tolerance || bb == 0.0) {
+      conv = true;
+      break;
+    }
+    if (inner_total >= opts.max_iters || outer >= 50) break;
+    ++outer;
+    // inner FP32 PCG on K d = r from d = 0
+    PA_HIP(hipMemsetAsync(h->m_d.p, 0, ndof * sizeof(float), s));
+    PA_HIP(pa::launch_pcg32_init(h->m_r.p, h->m_z.p, h->m_p.p, h->m_dinv.p, nn, h->partials.p, h->counter.p,
+                                 h->scal.p, s));
+    int it = 0;
+    while (inner_total < opts.max_iters) {
+      PA_HIP(pa::launch_mf_apply(ma, s));
+      PA_HIP(pa::launch_pcg32_update(h->m_d.p, h->m_r.p, h->m_z.p, h->m_p.p, h->m_q.p, h->m_dinv.p, nn,
+                                     h->partials.p, h->counter.p, h->scal.p, s));
+      PA_HIP(pa::launch_pcg32_pupdate(h->m_p.p, h->m_z.p, (int)ndof, h->scal.p, s));
+      ++it;
+      ++inner_total;
+      if (it % opts.check_every == 0) {
+        if (int rc = read_scal(h)) return rc;
+        const double ir = std::sqrt(h->host_scal[pa::kRR] / rr0);
+        if (!std::isfinite(ir)) return fail(PA_ERR_NOT_CONVERGED, "inner FP32 PCG: non-finite residual");
+        if (ir <= itol) break;
+      }
+    }
+    PA_HIP(pa::launch_axpy_f32(u, h->m_d.p, ndof, s));
+  }
+  if (st) {
+    st->iterations = inner_total;
+    st->outer_iterations = outer;
+    st->converged = conv ? 1 : 0;
+    st->rel_residual = rel;
+  }
+  if (!conv)
+    return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve stopped at relative residual " + std::to_string(rel));
+  return PA_OK;
+}
+
 int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_solver_opts* o,
                 pa_solver_stats* st) {
   if (int rc = ready(h)) return rc;
   if (!z || !u) return fail(PA_ERR_ARG, "pa_solution: null pointer");
   pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
-  if (opts.precision != 0) return fail(PA_ERR_UNSUPPORTED, "only precision 0 (FP64 PCG) is available");
+  if (opts.check_every <= 0) opts.check_every = 16;
+  if (opts.precision == 1) return solve_mixed(h, z, u, opts, st);
+  if (opts.precision != 0) return fail(PA_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
   if (!vals) return fail(PA_ERR_ARG, "pa_solution: FP64 solve needs the values buffer");
   if (int rc = ensure_solver_ws(h)) return rc;
   const size_t ndof = 3 * (size_t)h->nn;

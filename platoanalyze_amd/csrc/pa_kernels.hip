@@ -94,18 +94,16 @@ __device__ __forceinline__ double simp_pow(double rho, double p, int pint) {
   return pow(rho, p);
 }
 
-// 512-thread groups: ask for <=128 VGPRs so two groups (4 waves per SIMD) fit a CU.
+// Vector-valued element kernels (residual, dR/dz, energy and its gradient) over the
+// same chunks: node coordinates, density and state staged in LDS, elements streamed
+// as records of local node indices, owned rows accumulated in LDS.
 template <int BITS, int MODE, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
-void k_chunk(ChunkArgs A) {
+__global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   extern __shared__ double lds[];
-  constexpr bool kJac = MODE == kModeJacobian || MODE == kModeJacobianResidual;
-  constexpr bool kRes = MODE == kModeJacobianResidual || MODE == kModeResidual;
+  constexpr bool kRes = MODE == kModeResidual;
   constexpr bool kGzT = MODE == kModeGradZT;
-  constexpr bool kEnergy = MODE == kModeEnergy || MODE == kModeEnergyGradZ;
   constexpr bool kGradZ = MODE == kModeEnergyGradZ;
-  constexpr bool kNeedU = kRes || kGzT || kEnergy;
-  constexpr int kBlkW = kJac ? 9 : (kGzT ? 3 : 0);  // doubles per block in LDS
+  constexpr int kBlkW = kGzT ? 3 : 0;                 // doubles per block in LDS
   constexpr int kRowW = kRes ? 3 : (kGradZ ? 1 : 0);  // doubles per row in LDS
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
@@ -113,163 +111,121 @@ void k_chunk(ChunkArgs A) {
   const int nr = r1 - r0;
   const int b0 = A.rowmap[r0];
   const int nb = A.rowmap[r1] - b0;
-  double* seg = lds;
-  double* res = lds + (size_t)A.max_blocks * kBlkW;
+  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+  double4* nodes = reinterpret_cast<double4*>(lds);
+  double* ul = reinterpret_cast<double*>(nodes + A.max_nodes);
+  double* seg = ul + 3 * (size_t)A.max_nodes;
+  double* res = seg + (size_t)A.max_blocks * kBlkW;
   int* roff = reinterpret_cast<int*>(res + (size_t)A.max_rows * kRowW);
 
-  if constexpr (kBlkW > 0)
+  for (int k = threadIdx.x; k < nnode; k += NT) {
+    const size_t gid = (size_t)A.nodes[n0 + k];
+    nodes[k] = make_double4(A.X[3 * gid], A.X[3 * gid + 1], A.X[3 * gid + 2], A.z[gid]);
+    ul[3 * k + 0] = A.u[3 * gid + 0];
+    ul[3 * k + 1] = A.u[3 * gid + 1];
+    ul[3 * k + 2] = A.u[3 * gid + 2];
+  }
+  if constexpr (kBlkW > 0) {
     for (int k = threadIdx.x; k < nb * kBlkW; k += NT) seg[k] = 0.0;
+    for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
+  }
   if constexpr (kRowW > 0)
     for (int k = threadIdx.x; k < nr * kRowW; k += NT) res[k] = 0.0;
-  if constexpr (kBlkW > 0)
-    for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
   __syncthreads();
 
   double energy = 0.0;
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
-  // Lists are sorted by owned-vertex count, so the per-lane loops over owned
-  // vertices below have (nearly) the same trip count across a wavefront.
   for (int k = l0 + threadIdx.x; k < l1; k += NT) {
-    const uint32_t ent = A.list[k];
-    const int e = (int)(ent & 0x0FFFFFFFu);
-    const uint32_t mask = ent >> 28;
-    const int4 cn = A.conn[e];
-    const int nd[4] = {cn.x, cn.y, cn.z, cn.w};
-    double x[4][3], g[4][3], vol;
+    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+    double x[4][3], g[4][3], u[4][3], vol, rho = 0.0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a) {
+      const double4 p = nodes[ln[a]];
+      x[a][0] = p.x;
+      x[a][1] = p.y;
+      x[a][2] = p.z;
+      rho += p.w;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) x[a][d] = A.X[3 * (size_t)nd[a] + d];
+      for (int i = 0; i < 3; ++i) u[a][i] = ul[3 * ln[a] + i];
+    }
     tet_gradients(x, g, vol);
-    // ApplyWeighting: mean nodal density, MSIMP weight.
-    double rho = 0.0;
-#pragma unroll
-    for (int a = 0; a < 4; ++a) rho += A.z[nd[a]];
     rho = rho / 4.0;
     const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
+    double eps[6], s[6];
+    tet_strain(u, g, eps);
+    iso_stress(A.m, eps, s);
 
-    uint32_t cw[4];
-    if constexpr (kBlkW > 0) {
-      if constexpr (BITS == 4) {
-        const uint2 t = reinterpret_cast<const uint2*>(A.codes)[e];
-        cw[0] = t.x;
-        cw[1] = t.y;
-        cw[2] = t.y;
-        cw[3] = t.y;
-      } else {
-        const uint4 t = reinterpret_cast<const uint4*>(A.codes)[e];
-        cw[0] = t.x;
-        cw[1] = t.y;
-        cw[2] = t.z;
-        cw[3] = t.w;
-      }
-    }
-
-    double s[6], eps[6];
-    if constexpr (kNeedU) {
-      double u[4][3];
-#pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) u[a][d] = A.u[3 * (size_t)nd[a] + d];
-      tet_strain(u, g, eps);
-      iso_stress(A.m, eps, s);
-    }
-
-    double dw = 0.0;
-    if constexpr (kGradZ || kGzT)
-      dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
-
-    if constexpr (kEnergy) {
-      // ScalarProduct<6>(sigma_w, eps) * 0.5 * vol ; counted once, by the chunk owning vertex 0.
+    if constexpr (MODE == kModeEnergy || kGradZ) {
+      // ScalarProduct<6>(sigma_w, eps) * 0.5 * vol; counted by the chunk owning vertex 0
       double se = 0.0;
 #pragma unroll
       for (int q = 0; q < 6; ++q) se += (w * s[q]) * eps[q];
-      if (mask & 1u) energy += 0.5 * se * vol;
+      if (ln[0] < nr) energy += 0.5 * se * vol;
       if constexpr (kGradZ) {
         // d(0.5 vol sigma_w:eps)/dz_a = 0.5 vol (sigma:eps) * dw/drho * 1/4
         double sr = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) sr += s[q] * eps[q];
+        const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
         const double gz = 0.5 * vol * sr * dw * 0.25;
-        for (uint32_t bits = mask; bits; bits &= bits - 1) {
-          const int a = __builtin_ctz(bits);
-          lds_add(&res[sel4i(a, nd[0], nd[1], nd[2], nd[3]) - r0], gz);
-        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+          if (ln[a] < nr) lds_add(&res[ln[a]], gz);
       }
     }
-
-    double sw[6];
     if constexpr (kRes) {
+      double sw[6];
 #pragma unroll
       for (int q = 0; q < 6; ++q) sw[q] = w * s[q];
-    }
-    // closed-form block K_ab = L_a g_b^T + M_b g_a^T + (M_b.g_a) I with L = vw*lam*g,
-    // M = vw*mu*g (vw = vol * w)
-    const double vw = vol * w;
-    double rb[4][3];
-    if constexpr (kGzT) {
-      // dR_b/dz_a = dw/drho * 1/4 * (vol B_b^T sigma), rows = control node a.
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        tet_divergence(s, g[b], vol, rb[b]);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) rb[b][i] *= dw * 0.25;
-      }
-    }
-
-    if constexpr (kJac || kRes || kGzT) {
-      for (uint32_t bits = mask; bits; bits &= bits - 1) {
-        const int a = __builtin_ctz(bits);
-        const int lr = sel4i(a, nd[0], nd[1], nd[2], nd[3]) - r0;
-        double ga[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ga[i] = sel4(a, g[0][i], g[1][i], g[2][i], g[3][i]);
-        if constexpr (kRes) {
+      for (int a = 0; a < 4; ++a) {
+        if (ln[a] < nr) {
           double r[3];
-          tet_divergence(sw, ga, vol, r);
-          double* dst = &res[3 * lr];
+          tet_divergence(sw, g[a], vol, r);
+          double* dst = &res[3 * ln[a]];
           lds_add(dst + 0, r[0]);
           lds_add(dst + 1, r[1]);
           lds_add(dst + 2, r[2]);
         }
-        if constexpr (kJac || kGzT) {
-          const int base = roff[lr];
-          uint32_t code;
-          if constexpr (BITS == 4) code = (a < 2 ? cw[0] : cw[1]) >> (16 * (a & 1));
-          else code = a == 0 ? cw[0] : (a == 1 ? cw[1] : (a == 2 ? cw[2] : cw[3]));
-          if constexpr (kJac) {
-            const double vl = vw * A.m.c01, vm = vw * A.m.c33;
-            const double La[3] = {vl * ga[0], vl * ga[1], vl * ga[2]};
+      }
+    }
+    if constexpr (kGzT) {
+      // dR_b/dz_a = dw/drho * 1/4 * (vol B_b^T sigma); rows are the control nodes a
+      uint32_t cw[4];
+      if constexpr (BITS == 4) {
+        const uint2 t = reinterpret_cast<const uint2*>(A.ent_code)[k];
+        cw[0] = t.x;
+        cw[1] = t.y;
+        cw[2] = t.y;
+        cw[3] = t.y;
+      } else {
+        const uint4 t = reinterpret_cast<const uint4*>(A.ent_code)[k];
+        cw[0] = t.x;
+        cw[1] = t.y;
+        cw[2] = t.z;
+        cw[3] = t.w;
+      }
+      const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1) * 0.25;
+      double rb[4][3];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-              const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
-              double* dst = &seg[9 * sl];
-              const double Mb[3] = {vm * g[b][0], vm * g[b][1], vm * g[b][2]};
-              const double d = Mb[0] * ga[0] + Mb[1] * ga[1] + Mb[2] * ga[2];
+      for (int b = 0; b < 4; ++b) tet_divergence(s, g[b], vol, rb[b]);
 #pragma unroll
-              for (int i = 0; i < 3; ++i)
+      for (int a = 0; a < 4; ++a) {
+        if (ln[a] >= nr) continue;
+        const uint32_t code = BITS == 4 ? (cw[a >> 1] >> (16 * (a & 1))) : cw[a];
+        const int base = roff[ln[a]];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                  double v = La[i] * g[b][j] + Mb[i] * ga[j];
-                  if (i == j) v += d;
-                  lds_add(dst + 3 * i + j, v);
-                }
-            }
-          } else {
+        for (int b = 0; b < 4; ++b) {
+          const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-              const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
-#pragma unroll
-              for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], rb[b][i]);
-            }
-          }
+          for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], dw * rb[b][i]);
         }
       }
     }
   }
 
-  if constexpr (kEnergy) {
+  if constexpr (MODE == kModeEnergy || kGradZ) {
     // block reduction of the energy (fixed order -> deterministic)
     __shared__ double wsum[NT / 64];
     double v = energy;
@@ -295,7 +251,6 @@ void k_chunk(ChunkArgs A) {
       __syncthreads();
     }
   }
-
   if constexpr (kBlkW > 0) {
     double* dst = A.out_vals + (size_t)b0 * kBlkW;
     for (int k = threadIdx.x; k < nb * kBlkW; k += NT) dst[k] = seg[k];
@@ -305,7 +260,6 @@ void k_chunk(ChunkArgs A) {
     for (int k = threadIdx.x; k < nr * kRowW; k += NT) dst[k] = res[k];
   }
 }
-
 
 // Local edges (a,b), a<b, of a tet packed 4 bits per edge index t = 0..5.
 constexpr uint32_t kEdgeA = 0x211000u;  // 0,0,0,1,1,2
@@ -531,11 +485,11 @@ void k_jac_chunk(ChunkArgs A) {
   for (int k = threadIdx.x; k < nb * 9; k += NT) dst[k] = seg[k];
 }
 
-size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks) {
-  if (mode == kModeJacobian || mode == kModeJacobianResidual) return 0;  // see jac_lds_layout
-  int blkw = mode == kModeGradZT ? 3 : 0;
-  int roww = mode == kModeResidual ? 3 : (mode == kModeEnergyGradZ ? 1 : 0);
-  size_t bytes = sizeof(double) * ((size_t)max_blocks * blkw + (size_t)max_rows * roww);
+size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
+  if (mode == kModeJacobian || mode == kModeJacobianResidual) return jac_lds_layout(max_rows, max_blocks, max_nodes).total;
+  const int blkw = mode == kModeGradZT ? 3 : 0;
+  const int roww = mode == kModeResidual ? 3 : (mode == kModeEnergyGradZ ? 1 : 0);
+  size_t bytes = sizeof(double) * (7 * (size_t)max_nodes + (size_t)max_blocks * blkw + (size_t)max_rows * roww);
   if (blkw > 0) bytes += sizeof(int) * (size_t)(max_rows + 1);
   return bytes;
 }
@@ -547,9 +501,7 @@ constexpr int kNtVec = 256;
 
 template <int BITS>
 static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
-  size_t lds = chunk_lds_bytes(mode, a.max_rows, a.max_blocks);
-  if (mode == kModeJacobian || mode == kModeJacobianResidual)
-    lds = jac_lds_layout(a.max_rows, a.max_blocks, a.max_nodes).total;
+  const size_t lds = chunk_lds_bytes(mode, a.max_rows, a.max_blocks, a.max_nodes);
   dim3 grid(a.nchunks);
   switch (mode) {
     case kModeJacobian: k_jac_chunk<BITS, false, kNtJac><<<grid, kNtJac, lds, s>>>(a); break;

@@ -61,7 +61,7 @@ enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2
 
 
 // launch helpers (defined in pa_kernels.hip)
-size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks);
+size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes);
 hipError_t launch_chunk(int bits, int mode, const ChunkArgs& a, hipStream_t s);
 
 }  // namespace pa

@@ -1,0 +1,400 @@
+// Matrix-free FP32 operator, FP64 block-diagonal preconditioner and FP32 PCG steps
+// for the mixed-precision solve (inner FP32 PCG, outer FP64 iterative refinement).
+#include "pa_mf.hpp"
+#include "pa_solver.hpp"
+
+namespace pa {
+
+namespace {
+
+constexpr int kT = 256;
+
+__device__ __forceinline__ int xcd_map(int b, int n) {
+  const int q = n / kNumXcd, r = n % kNumXcd;
+  const int x = b % kNumXcd, k = b / kNumXcd;
+  return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int q = 0; q < (int)(blockDim.x / 64); ++q) t += sh[q];
+  return t;
+}
+
+__device__ __forceinline__ bool last_block(unsigned* counter) {
+  __shared__ int am_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (am_last) __threadfence();
+  return am_last != 0;
+}
+
+__device__ __forceinline__ float powi_f(float x, float p, int pint) {
+  if (pint > 0) {
+    float r = x;
+    for (int k = 1; k < pint; ++k) r *= x;
+    return r;
+  }
+  return powf(x, p);
+}
+
+template <typename T>
+__device__ __forceinline__ void gradients_t(const T x[4][3], T g[4][3], T& vol) {
+  const T j11 = x[0][0] - x[3][0], j12 = x[1][0] - x[3][0], j13 = x[2][0] - x[3][0];
+  const T j21 = x[0][1] - x[3][1], j22 = x[1][1] - x[3][1], j23 = x[2][1] - x[3][1];
+  const T j31 = x[0][2] - x[3][2], j32 = x[1][2] - x[3][2], j33 = x[2][2] - x[3][2];
+  const T det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 - j12 * j21 * j33 -
+                j13 * j22 * j31;
+  const T inv = T(1) / det;
+  g[0][0] = (j22 * j33 - j23 * j32) * inv;
+  g[0][1] = (j13 * j32 - j12 * j33) * inv;
+  g[0][2] = (j12 * j23 - j13 * j22) * inv;
+  g[1][0] = (j23 * j31 - j21 * j33) * inv;
+  g[1][1] = (j11 * j33 - j13 * j31) * inv;
+  g[1][2] = (j13 * j21 - j11 * j23) * inv;
+  g[2][0] = (j21 * j32 - j22 * j31) * inv;
+  g[2][1] = (j12 * j31 - j11 * j32) * inv;
+  g[2][2] = (j11 * j22 - j12 * j21) * inv;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) g[3][d] = -(g[0][d] + g[1][d] + g[2][d]);
+  vol = fabs(det) * (T(1) / T(6));
+}
+
+// y = K x (FP32): chunk node table with FP32 coordinates/density and the input
+// vector staged in LDS; residual-style element loop; ds_add_f32 into the owned rows.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
+  extern __shared__ float smem[];
+  float4* nd4 = reinterpret_cast<float4*>(smem);
+  float* xl = reinterpret_cast<float*>(nd4 + A.max_nodes);
+  float* acc = xl + 3 * A.max_nodes;
+  __shared__ double sh[NT / 64];
+
+  const int c = xcd_map(blockIdx.x, A.nchunks);
+  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
+  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+  for (int k = threadIdx.x; k < nnode; k += NT) {
+    const int gid = A.nodes[n0 + k];
+    nd4[k] = A.Xz[gid];
+    const uint32_t bm = A.bcmask[gid];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) xl[3 * k + i] = (bm >> i) & 1u ? 0.0f : A.x[3 * (size_t)gid + i];
+  }
+  for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0f;
+  __syncthreads();
+
+  const float c00 = (float)A.m.c00, c01 = (float)A.m.c01, c33 = (float)A.m.c33;
+  const float rmin = (float)A.m.rmin, pp = (float)A.m.p;
+  const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
+    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+    float x[4][3], g[4][3], u[4][3], vol, rho = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const float4 p = nd4[ln[a]];
+      x[a][0] = p.x;
+      x[a][1] = p.y;
+      x[a][2] = p.z;
+      rho += p.w;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u[a][i] = xl[3 * ln[a] + i];
+    }
+    gradients_t(x, g, vol);
+    rho *= 0.25f;
+    const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
+    float e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      e[0] += u[a][0] * g[a][0];
+      e[1] += u[a][1] * g[a][1];
+      e[2] += u[a][2] * g[a][2];
+      e[3] += u[a][2] * g[a][1] + u[a][1] * g[a][2];
+      e[4] += u[a][2] * g[a][0] + u[a][0] * g[a][2];
+      e[5] += u[a][1] * g[a][0] + u[a][0] * g[a][1];
+    }
+    const float s0 = vw * (c00 * e[0] + c01 * e[1] + c01 * e[2]);
+    const float s1 = vw * (c01 * e[0] + c00 * e[1] + c01 * e[2]);
+    const float s2 = vw * (c01 * e[0] + c01 * e[1] + c00 * e[2]);
+    const float s3 = vw * c33 * e[3], s4 = vw * c33 * e[4], s5 = vw * c33 * e[5];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (ln[a] < nr) {
+        float* dst = &acc[3 * ln[a]];
+        atomicAdd(dst + 0, s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]);
+        atomicAdd(dst + 1, s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]);
+        atomicAdd(dst + 2, s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]);
+      }
+    }
+  }
+  __syncthreads();
+  double dot = 0.0;
+  for (int k = threadIdx.x; k < 3 * nr; k += NT) {
+    const size_t gd = 3 * (size_t)r0 + k;
+    const float xv = A.x[gd];
+    const float yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? xv : acc[k];
+    A.y[gd] = yv;
+    dot += (double)xv * (double)yv;
+  }
+  const double t = block_sum(dot, sh);
+  if (threadIdx.x == 0) A.partials[c] = t;
+  if (last_block(A.counter)) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < A.nchunks; i += NT) v += A.partials[i];
+    const double tot = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+      if (A.scal_out >= 0) A.scal[A.scal_out] = tot;
+      if (A.scal_num >= 0) A.scal[kAlpha] = A.scal[A.scal_num] / tot;
+      *A.counter = 0;
+    }
+  }
+}
+
+// Diagonal blocks K_rr (FP64) accumulated per owned vertex, then inverted per row.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
+  extern __shared__ double dsm[];
+  double* D = dsm;  // 9 per row
+  const int c = xcd_map(blockIdx.x, A.nchunks);
+  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
+  const int n0 = A.node_offs[c];
+  for (int k = threadIdx.x; k < 9 * nr; k += NT) D[k] = 0.0;
+  __syncthreads();
+  const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
+    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+    double x[4][3], g[4][3], vol, rho = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const size_t gid = (size_t)A.nodes[n0 + ln[a]];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) x[a][d] = A.X[3 * gid + d];
+      rho += A.z[gid];
+    }
+    gradients_t(x, g, vol);
+    rho = rho / 4.0;
+    double pw;
+    if (A.m.pint > 0) {
+      pw = rho;
+      for (int q = 1; q < A.m.pint; ++q) pw *= rho;
+    } else {
+      pw = pow(rho, A.m.p);
+    }
+    const double vw = vol * (A.m.rmin + (1.0 - A.m.rmin) * pw);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (ln[a] >= nr) continue;
+      const double gg = g[a][0] * g[a][0] + g[a][1] * g[a][1] + g[a][2] * g[a][2];
+      double* dst = &D[9 * ln[a]];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          double v = (A.m.c01 + A.m.c33) * g[a][i] * g[a][j];
+          if (i == j) v += A.m.c33 * gg;
+          __hip_atomic_fetch_add(dst + 3 * i + j, vw * v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+  }
+  __syncthreads();
+  for (int lr = threadIdx.x; lr < nr; lr += NT) {
+    double a[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) a[q] = D[9 * lr + q];
+    const uint32_t bm = A.bcmask[r0 + lr];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        if (((bm >> i) | (bm >> j)) & 1u) a[3 * i + j] = (i == j) ? 1.0 : 0.0;
+    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+    const double id = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
+    float* o = A.dinv + 9 * (size_t)(r0 + lr);
+    o[0] = (float)(c00 * id);
+    o[1] = (float)((a[2] * a[7] - a[1] * a[8]) * id);
+    o[2] = (float)((a[1] * a[5] - a[2] * a[4]) * id);
+    o[3] = (float)(c01 * id);
+    o[4] = (float)((a[0] * a[8] - a[2] * a[6]) * id);
+    o[5] = (float)((a[2] * a[3] - a[0] * a[5]) * id);
+    o[6] = (float)(c02 * id);
+    o[7] = (float)((a[1] * a[6] - a[0] * a[7]) * id);
+    o[8] = (float)((a[0] * a[4] - a[1] * a[3]) * id);
+  }
+}
+
+__device__ __forceinline__ void dinv_apply(const float* dinv, const float* r, float* z, int n) {
+  const float* D = dinv + 9 * (size_t)n;
+  const float a = r[3 * (size_t)n], b = r[3 * (size_t)n + 1], cc = r[3 * (size_t)n + 2];
+  z[3 * (size_t)n + 0] = D[0] * a + D[1] * b + D[2] * cc;
+  z[3 * (size_t)n + 1] = D[3] * a + D[4] * b + D[5] * cc;
+  z[3 * (size_t)n + 2] = D[6] * a + D[7] * b + D[8] * cc;
+}
+
+__global__ __launch_bounds__(kT) void k_pcg32_init(const float* r, float* z, float* p, const float* dinv, int nn,
+                                                   double* partials, unsigned* counter, double* scal) {
+  __shared__ double sh[kT / 64];
+  double rz = 0.0;
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+    dinv_apply(dinv, r, z, n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)n + i;
+      p[d] = z[d];
+      rz += (double)r[d] * (double)z[d];
+    }
+  }
+  const double t = block_sum(rz, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  if (last_block(counter)) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) v += partials[i];
+    const double tot = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+      scal[kRZ] = tot;
+      *counter = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_pcg32_update(float* x, float* r, float* z, const float* p, const float* q,
+                                                     const float* dinv, int nn, double* partials, unsigned* counter,
+                                                     double* scal) {
+  __shared__ double sh[kT / 64];
+  const float alpha = (float)scal[kAlpha];
+  double rz = 0.0, rr = 0.0;
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)n + i;
+      x[d] += alpha * p[d];
+      r[d] -= alpha * q[d];
+    }
+    dinv_apply(dinv, r, z, n);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)n + i;
+      rz += (double)r[d] * (double)z[d];
+      rr += (double)r[d] * (double)r[d];
+    }
+  }
+  const double t0 = block_sum(rz, sh);
+  const double t1 = block_sum(rr, sh);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = t0;
+    partials[2 * blockIdx.x + 1] = t1;
+  }
+  if (last_block(counter)) {
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) {
+      a += partials[2 * i];
+      b += partials[2 * i + 1];
+    }
+    const double sa = block_sum(a, sh);
+    const double sb = block_sum(b, sh);
+    if (threadIdx.x == 0) {
+      scal[kBeta] = sa / scal[kRZ];
+      scal[kRZ] = sa;
+      scal[kRR] = sb;
+      *counter = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_pcg32_pupdate(float* p, const float* z, int ndof, const double* scal) {
+  const float beta = (float)scal[kBeta];
+  for (int d = blockIdx.x * kT + threadIdx.x; d < ndof; d += gridDim.x * kT) p[d] = z[d] + beta * p[d];
+}
+
+__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const uint8_t* bcmask, float* r32, int nn,
+                                                     double* partials, unsigned* counter, double* scal, int slot) {
+  __shared__ double sh[kT / 64];
+  double rr = 0.0;
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+    const uint32_t bm = bcmask[n];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)n + i;
+      const double v = (bm >> i) & 1u ? 0.0 : -R[d];
+      r32[d] = (float)v;
+      rr += v * v;
+    }
+  }
+  const double t = block_sum(rr, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+  if (last_block(counter)) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) v += partials[i];
+    const double tot = block_sum(v, sh);
+    if (threadIdx.x == 0) {
+      scal[slot] = tot;
+      *counter = 0;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_axpy_f32(double* u, const float* d, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) u[i] += (double)d[i];
+}
+
+__global__ __launch_bounds__(kT) void k_make_xz(const double* X, const double* z, float4* Xz, int nn) {
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT)
+    Xz[n] = make_float4((float)X[3 * (size_t)n], (float)X[3 * (size_t)n + 1], (float)X[3 * (size_t)n + 2],
+                        (float)z[n]);
+}
+
+int grid_of(size_t n, int cap = 1024) {
+  size_t g = (n + kT - 1) / kT;
+  return (int)(g < 1 ? 1 : (g > (size_t)cap ? cap : g));
+}
+
+}  // namespace
+
+hipError_t launch_mf_apply(const MfArgs& a, hipStream_t s) {
+  const size_t lds = sizeof(float4) * a.max_nodes + sizeof(float) * 3 * (a.max_nodes + a.max_rows);
+  k_mf_apply<256><<<a.nchunks, 256, lds, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mf_diag_inv(const DiagArgs& a, hipStream_t s) {
+  k_mf_diag<256><<<a.nchunks, 256, sizeof(double) * 9 * a.max_rows, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pcg32_init(const float* r, float* z, float* p, const float* dinv, int nn, double* partials,
+                             unsigned* counter, double* scal, hipStream_t s) {
+  k_pcg32_init<<<grid_of(nn), kT, 0, s>>>(r, z, p, dinv, nn, partials, counter, scal);
+  return hipGetLastError();
+}
+hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, const float* q, const float* dinv, int nn,
+                               double* partials, unsigned* counter, double* scal, hipStream_t s) {
+  k_pcg32_update<<<grid_of(nn), kT, 0, s>>>(x, r, z, p, q, dinv, nn, partials, counter, scal);
+  return hipGetLastError();
+}
+hipError_t launch_pcg32_pupdate(float* p, const float* z, int ndof, const double* scal, hipStream_t s) {
+  k_pcg32_pupdate<<<grid_of(ndof), kT, 0, s>>>(p, z, ndof, scal);
+  return hipGetLastError();
+}
+hipError_t launch_neg_mask_to_f32(const double* R, const uint8_t* bcmask, float* r32, int nn, double* partials,
+                                  unsigned* counter, double* scal, int slot, hipStream_t s) {
+  k_neg_mask_f32<<<grid_of(nn), kT, 0, s>>>(R, bcmask, r32, nn, partials, counter, scal, slot);
+  return hipGetLastError();
+}
+hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nn, hipStream_t s) {
+  k_make_xz<<<grid_of(nn, 4096), kT, 0, s>>>(X, z, Xz, nn);
+  return hipGetLastError();
+}
+hipError_t launch_axpy_f32(double* u, const float* d, size_t n, hipStream_t s) {
+  k_axpy_f32<<<grid_of(n, 4096), kT, 0, s>>>(u, d, n);
+  return hipGetLastError();
+}
+
+}  // namespace pa

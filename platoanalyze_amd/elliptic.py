@@ -204,20 +204,34 @@ class Problem:
     """Plato::Elliptic::Problem (self-adjoint compliance)."""
 
     def __init__(self, model: ElasticModel, tolerance: float = 1e-10, max_iters: int = 200000,
-                 check_every: int = 16):
+                 check_every: int = 16, precision: str = "fp64", inner_tolerance: float = 1e-4):
+        """precision "fp64": assembled block-CRS Jacobi-PCG in FP64.  "mixed": FP64
+        iterative refinement (true residual f - K u) around an FP32 matrix-free
+        Jacobi-PCG solving each correction to `inner_tolerance`."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
         self.m = model
-        self.opts = _capi.SolverOpts(tolerance, max_iters, check_every, 0, 0, 1e-4)
+        self.opts = _capi.SolverOpts(tolerance, max_iters, check_every, 0 if precision == "fp64" else 1, 0,
+                                     inner_tolerance)
         self.stats = _capi.SolverStats()
         self._vals = None
         self.state = None
 
-    def solution(self, z: torch.Tensor) -> torch.Tensor:
-        u = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device)
-        if self._vals is None:
-            self._vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+    def solution(self, z: torch.Tensor, u0: torch.Tensor | None = None) -> torch.Tensor:
+        """Problem::solution; `u0` warm-starts the solve (mixed precision only)."""
+        if u0 is not None:
+            u = u0.detach().clone().contiguous()
+            self.opts.use_x0 = 1
+        else:
+            u = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device)
+            self.opts.use_x0 = 0
+        vals = None
+        if self.opts.precision == 0:
+            if self._vals is None:
+                self._vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+            vals = _dptr(self._vals)
         self.m._stream()
-        _capi.call("pa_solution", self.m.handle, _dptr(z), _dptr(u), _dptr(self._vals), C.byref(self.opts),
-                   C.byref(self.stats))
+        _capi.call("pa_solution", self.m.handle, _dptr(z), _dptr(u), vals, C.byref(self.opts), C.byref(self.stats))
         self.state = u
         return u
 

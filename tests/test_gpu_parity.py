@@ -282,3 +282,49 @@ def test_gpu_scale_cfg1_parity():
     sf = PhysicsScalarFunction(m)
     assert abs(sf.value(du, dz) - v) <= RTOL * abs(v)
     close(host(sf.gradient_z(du, dz)), gz, RTOL, "energy grad_z")
+
+
+# ------------------------------------------------------------------ mixed precision
+def test_gpu_mixed_precision_solve_cfg0():
+    """FP32 matrix-free PCG inside FP64 iterative refinement reaches the FP64 tolerance;
+    errors against the FP64 block-CRS PCG solution of the same system are bounded."""
+    print("case: cfg0 mixed precision", flush=True)
+    w = wl.make("cantilever_128x64x64", tiny=(20, 10, 10))
+    m = wl.model(w)
+    dz = dev(w.z)
+    ref = Problem(m, tolerance=1e-12)
+    u64 = host(ref.solution(dz))
+    g64, C64 = ref.criterionGradient(dz, dev(u64))
+    g64 = host(g64)
+    mp = Problem(m, tolerance=1e-10, precision="mixed", inner_tolerance=1e-4)
+    u = mp.solution(dz)
+    assert mp.stats.converged and mp.stats.rel_residual <= 1e-10
+    assert mp.stats.outer_iterations >= 2  # refinement actually iterated
+    uh = host(u)
+    err_u = np.abs(uh - u64).max() / np.abs(u64).max()
+    g, C = mp.criterionGradient(dz, u)
+    err_c = abs(C - C64) / abs(C64)
+    err_g = np.abs(host(g) - g64).max() / np.abs(g64).max()
+    print(f"mixed vs fp64: u {err_u:.2e} compliance {err_c:.2e} dC/dz {err_g:.2e} "
+          f"outer {mp.stats.outer_iterations} inner {mp.stats.iterations}", flush=True)
+    assert err_u < 1e-7 and err_c < 1e-9 and err_g < 1e-7
+    # warm start from the converged state: already converged, no inner iteration
+    mp2 = Problem(m, tolerance=1e-10, precision="mixed")
+    mp2.solution(dz, u0=u)
+    assert mp2.stats.iterations == 0 and mp2.stats.converged
+
+
+def test_gpu_mixed_precision_nonzero_dirichlet():
+    mesh = build_box(8, 4, 4, 2.0, 1.0, 1.0)
+    fixed = np.flatnonzero(mesh.coords[:, 0] == 0.0).astype(np.int32)
+    bcs = [EssentialBC(fixed, 0, 0.0), EssentialBC(fixed, 1, 0.01), EssentialBC(fixed, 2, -0.02)]
+    from platoanalyze_amd.mesh import faces_on_plane
+    fv, fe = faces_on_plane(mesh, 0, 2.0)
+    m = ElasticModel(mesh, 1.0, 0.3, 3.0, 1e-3, bcs, [Traction(fv, fe, (0.0, 0.0, -1.0))])
+    z = dev(np.random.default_rng(9).uniform(0.2, 1.0, mesh.num_nodes))
+    u64 = host(Problem(m, tolerance=1e-12).solution(z))
+    mp = Problem(m, tolerance=1e-10, precision="mixed")
+    uh = host(mp.solution(z))
+    assert mp.stats.converged
+    np.testing.assert_allclose(uh[3 * fixed + 1], 0.01, rtol=0, atol=0)
+    assert np.abs(uh - u64).max() <= 1e-7 * np.abs(u64).max()
