@@ -168,6 +168,30 @@ int pa_solution(pa_handle h, const double* z, double* u, double* values,
 int pa_compliance_gradient(pa_handle h, const double* u, const double* z, double* dcdz,
                            double* value);
 
+/* ------------------------------------------------- stress p-norm criterion and adjoint
+ * F = ( sum_e vol_e * m_e^p )^(1/p) with
+ *   measure 0: m_e = Voigt norm of the SIMP-weighted stress -- the reference
+ *              "Stress P-Norm" (src/elliptic/StressPNorm.hpp:88-142, TensorPNorm
+ *              src/TensorPNorm.hpp:28-45,286-318; p is truncated to an integer as there);
+ *   measure 1: m_e = rho_e^q * vonMises(sigma0_e), the relaxed von Mises stress of the
+ *              unpenalized stress (vonMises as src/VonMisesYieldFunction.hpp:53-74).
+ * pa_stress_pnorm: value (host) and the partial derivatives dF/du (3N) and dF/dz (N),
+ * either may be NULL (PhysicsScalarFunction::value / gradient_u / gradient_z). */
+int pa_stress_pnorm(pa_handle h, const double* u, const double* z, int measure, double p, double q,
+                    double* value, double* dfdu, double* dfdz);
+
+/* (dR/dz)^T lambda per control node (VectorFunction::gradient_z applied transposed,
+ * the MatrixTimesVectorPlusVector term of src/elliptic/Problem.hpp:437-466) without
+ * forming dR/dz. */
+int pa_dRdz_T_apply(pa_handle h, const double* u, const double* lambda, const double* z, double* out);
+
+/* Problem::criterionGradient, non-self-adjoint branch (src/elliptic/Problem.hpp:413-468):
+ * K lambda = dF/du (homogeneous constraints; FP64 CRS PCG or mixed precision per opts),
+ * dF/dz = dF/dz|_u - (dR/dz)^T lambda.  `lambda` (3N) receives the adjoint state. */
+int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int measure, double p, double q,
+                             double* dfdz, double* lambda, double* value, const pa_solver_opts* opts,
+                             pa_solver_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

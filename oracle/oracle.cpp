@@ -107,6 +107,23 @@ inline Fad<N> fad_pow(const Fad<N>& a, double c) {
   return r;
 }
 inline double fad_pow(double a, double c) { return std::pow(a, c); }
+template <int N>
+inline Fad<N> fad_sqrt(const Fad<N>& a) {
+  Fad<N> r(std::sqrt(a.v));
+  const double d = a.v > 0.0 ? 0.5 / r.v : 0.0;
+  for (int i = 0; i < N; ++i) r.dx[i] = a.dx[i] * d;
+  return r;
+}
+inline double fad_sqrt(double a) { return std::sqrt(a); }
+template <int N>
+inline Fad<N> operator*(const Fad<N>& a, int b) { return a * (double)b; }
+// product of a stress-typed and a weight-typed value into the result type
+template <typename R, typename A, typename B>
+inline R mix(const A& a, const B& b) { return R(a * b); }
+template <>
+inline Fad<12> mix<Fad<12>, Fad<12>, double>(const Fad<12>& a, const double& b) { return a * b; }
+template <>
+inline Fad<4> mix<Fad<4>, double, Fad<4>>(const double& a, const Fad<4>& b) { return b * a; }
 
 template <typename T>
 struct ValueOf {
@@ -642,18 +659,59 @@ int orc_pcg(int nn, const int* rowmap, const int* col, const double* A, const do
   return it;
 }
 
-// StressPNorm (src/elliptic/StressPNorm.hpp:88-142) with TensorPNorm
-// (src/TensorPNorm.hpp:28-45, 286-318) in its default (unit-weight Voigt) form:
-// F = ( sum_e vol_e * (sum_k sigma_w[k]^2)^(p/2) )^(1/p), where the functor receives
-// the exponent as an integer (TensorPNormFunctor's `Plato::OrdinalType p`).
-double orc_stress_pnorm(int nn, int ne, const double* X, const int* conn, const double* u, const double* z,
-                        double E, double nu, double p, double rmin, double pnorm) {
+}  // extern "C"
+
+// Stress p-norm criteria, F = ( sum_e vol_e * m_e^p )^(1/p):
+//   measure 0: the reference StressPNorm (src/elliptic/StressPNorm.hpp:88-142) with
+//     TensorPNorm (src/TensorPNorm.hpp:28-45, 286-318): m_e = |sigma_w|_Voigt, the
+//     unit-weight Voigt norm of the SIMP-weighted stress; the functor receives the
+//     exponent as an integer (TensorPNormFunctor's `Plato::OrdinalType p`).
+//   measure 1: relaxed von Mises, m_e = rho_e^q vm(sigma0_e) with the unpenalized
+//     stress sigma0 and vm of VonMisesYieldFunction (src/VonMisesYieldFunction.hpp:53-74).
+// Derivatives through forward-mode dual numbers (12 state / 4 control slots) and the
+// postEvaluate scaling pow(S,(1-p)/p)/p (TensorNormBase::postEvaluate).
+template <typename T>
+T vm_of(const T* s) {
+  T d01 = s[0] - s[1], d12 = s[1] - s[2], d20 = s[2] - s[0];
+  T j2 = 0.5 * (d01 * d01 + d12 * d12 + d20 * d20) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  return fad_sqrt(j2);
+}
+
+template <typename R, typename S, typename Z>
+R pnorm_cell(const Params& P, const S* eps, const Z* zc, double vol, int measure, double pn, double q) {
+  S s[6];
+  stress(P, eps, s);
+  Z rho(0.0);
+  for (int n = 0; n < 4; ++n) rho += zc[n];
+  rho = rho / 4.0;
+  if (measure == 0) {
+    Z w = P.rmin + (1.0 - P.rmin) * fad_pow(rho, P.p);
+    R ss(0.0);
+    for (int k = 0; k < 6; ++k) {
+      R t = mix<R>(s[k], w);
+      ss += t * t;
+    }
+    return fad_pow(ss, (int)pn / 2.0) * vol;
+  }
+  Z rq = fad_pow(rho, q);
+  R sr[6];
+  for (int k = 0; k < 6; ++k) sr[k] = mix<R>(s[k], rq);
+  R m = vm_of(sr);
+  return fad_pow(m, pn) * vol;
+}
+
+extern "C" {
+
+// value; grad_u (3N) and grad_z (N) are the partial derivatives of F (may be NULL)
+double orc_pnorm(int nn, int ne, const double* X, const int* conn, const double* u, const double* z, double E,
+                 double nu, double p, double rmin, int measure, double pn, double q, double* grad_u,
+                 double* grad_z) {
   Mesh m{nn, ne, X, conn};
   Params P;
   make_params(E, nu, p, rmin, P);
   auto cfg = workset_config(m);
-  double total = 0.0;
-#pragma omp parallel for schedule(static) reduction(+ : total)
+  double S = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : S)
   for (int e = 0; e < ne; ++e) {
     double g[12], v, ue[12], zc[4];
     gradient_workset(&cfg[12 * (size_t)e], g, v);
@@ -661,15 +719,91 @@ double orc_stress_pnorm(int nn, int ne, const double* X, const int* conn, const 
       zc[n] = z[conn[4 * e + n]];
       for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
     }
+    double eps[6];
+    strain(ue, g, eps);
+    S += pnorm_cell<double, double, double>(P, eps, zc, v, measure, pn, q);
+  }
+  const double pe = measure == 0 ? (double)(int)pn : pn;
+  const double scale = std::pow(S, (1.0 - pe) / pe) / pe;
+  if (grad_u) {
+    using F = Fad<12>;
+    std::memset(grad_u, 0, sizeof(double) * 3 * (size_t)nn);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+      double g[12], v, zc[4];
+      gradient_workset(&cfg[12 * (size_t)e], g, v);
+      F ue[12];
+      for (int n = 0; n < 4; ++n) {
+        zc[n] = z[conn[4 * e + n]];
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = F(u[3 * (size_t)conn[4 * e + n] + d], 3 * n + d);
+      }
+      F eps[6];
+      strain(ue, g, eps);
+      F val = pnorm_cell<F, F, double>(P, eps, zc, v, measure, pn, q);
+      for (int k = 0; k < 12; ++k)
+        atomic_add(&grad_u[3 * (size_t)conn[4 * e + k / 3] + k % 3], scale * val.dx[k]);
+    }
+  }
+  if (grad_z) {
+    using F = Fad<4>;
+    std::memset(grad_z, 0, sizeof(double) * (size_t)nn);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+      double g[12], v, ue[12];
+      gradient_workset(&cfg[12 * (size_t)e], g, v);
+      F zc[4];
+      for (int n = 0; n < 4; ++n) {
+        zc[n] = F(z[conn[4 * e + n]], n);
+        for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
+      }
+      double eps[6];
+      strain(ue, g, eps);
+      F val = pnorm_cell<F, double, F>(P, eps, zc, v, measure, pn, q);
+      for (int n = 0; n < 4; ++n) atomic_add(&grad_z[conn[4 * e + n]], scale * val.dx[n]);
+    }
+  }
+  return std::pow(S, 1.0 / pe);
+}
+
+double orc_stress_pnorm(int nn, int ne, const double* X, const int* conn, const double* u, const double* z,
+                        double E, double nu, double p, double rmin, double pnorm) {
+  return orc_pnorm(nn, ne, X, conn, u, z, E, nu, p, rmin, 0, pnorm, 0.0, nullptr, nullptr);
+}
+
+// (dR/dz)^T lambda per control node: sum_e dw/drho/4 * lambda_e . (vol B^T sigma0(u_e))
+// (VectorFunction::gradient_z transposed, applied to a vector: src/PlatoMathHelpers.hpp:58-138)
+void orc_dRdz_T_apply(int nn, int ne, const double* X, const int* conn, const double* u, const double* lam,
+                      const double* z, double E, double nu, double p, double rmin, double* out) {
+  Mesh m{nn, ne, X, conn};
+  Params P;
+  make_params(E, nu, p, rmin, P);
+  auto cfg = workset_config(m);
+  std::memset(out, 0, sizeof(double) * (size_t)nn);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < ne; ++e) {
+    double g[12], v, ue[12], le[12];
+    gradient_workset(&cfg[12 * (size_t)e], g, v);
+    Fad<4> zc[4];
+    for (int n = 0; n < 4; ++n) {
+      zc[n] = Fad<4>(z[conn[4 * e + n]], n);
+      for (int d = 0; d < 3; ++d) {
+        ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
+        le[3 * n + d] = lam[3 * (size_t)conn[4 * e + n] + d];
+      }
+    }
     double eps[6], s[6];
     strain(ue, g, eps);
     stress(P, eps, s);
-    double w = msimp_weight(P, zc);
-    double ss = 0.0;
-    for (int k = 0; k < 6; ++k) ss += (s[k] * w) * (s[k] * w);
-    total += std::pow(ss, (int)pnorm / 2.0) * v;
+    Fad<4> w = msimp_weight(P, zc);
+    Fad<4> sw[6], r[12];
+    for (int k = 0; k < 6; ++k) sw[k] = w * s[k];
+    divergence(sw, g, v, r);
+    for (int n = 0; n < 4; ++n) {
+      double acc = 0.0;
+      for (int k = 0; k < 12; ++k) acc += le[k] * r[k].dx[n];
+      atomic_add(&out[conn[4 * e + n]], acc);
+    }
   }
-  return std::pow(total, 1.0 / pnorm);
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@ def lib():
             raise ImportError(f"{LIB} not built (python -m platoanalyze_amd.build)")
         _lib = C.CDLL(str(LIB))
         _lib.orc_stress_pnorm.restype = C.c_double
+        _lib.orc_pnorm.restype = C.c_double
     return _lib
 
 
@@ -106,6 +107,21 @@ class Model:
         lib().orc_cell_worksets(*self._m(), _p(_f64(u)), C.c_double(self.E), C.c_double(self.nu), ncells,
                                 _p(g), _p(vol), _p(eps), _p(sig), _p(res))
         return g, vol, eps, sig, res
+
+    def pnorm(self, u, z, measure, pn, q=0.0, want_grad_u=True, want_grad_z=True):
+        """F = (sum vol m^p)^(1/p); measure 0 = reference StressPNorm (Voigt norm of the
+        SIMP-weighted stress), 1 = relaxed von Mises rho^q vm(sigma0).  Returns F and the
+        partial derivatives dF/du, dF/dz (None when not requested)."""
+        gu = np.empty(3 * self.nn) if want_grad_u else None
+        gz = np.empty(self.nn) if want_grad_z else None
+        v = lib().orc_pnorm(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), int(measure), C.c_double(pn),
+                            C.c_double(q), _p(gu), _p(gz))
+        return float(v), gu, gz
+
+    def dRdz_T_apply(self, u, lam, z):
+        out = np.empty(self.nn)
+        lib().orc_dRdz_T_apply(*self._m(), _p(_f64(u)), _p(_f64(lam)), _p(_f64(z)), *self._mat(), _p(out))
+        return out
 
     def stress_pnorm(self, u, z, pnorm):
         return float(lib().orc_stress_pnorm(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(),

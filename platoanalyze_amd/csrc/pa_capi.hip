@@ -103,7 +103,7 @@ struct pa_problem {
   DevArray<double> scal;
   DevArray<double> chunk_partials;
   // solver workspace (lazily allocated)
-  DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals;
+  DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -577,7 +577,10 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
 // Mixed precision: FP64 iterative refinement around an FP32 matrix-free Jacobi-PCG.
 // Outer: r = f - K u in FP64 (element residual kernel, constrained dofs masked);
 // inner: K d = r to inner_tolerance with y = K x in FP32; u += d in FP64.
-static int solve_mixed(pa_handle h, const double* z, double* u, const pa_solver_opts& opts, pa_solver_stats* st) {
+// b == nullptr: right-hand side is the traction load f; else b (device, 3N; constrained
+// entries ignored).
+static int solve_mixed(pa_handle h, const double* z, double* u, const double* b, const pa_solver_opts& opts,
+                       pa_solver_stats* st) {
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
   hipStream_t s = h->stream;
@@ -609,7 +612,8 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const pa_solver_
   PA_HIP(pa::launch_mf_diag_inv(da, s));
   if (!opts.use_x0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(u, ndof, 0.0);
   const int nbc = (int)h->bc_dofs.size();
-  if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(u, h->bc_d.p, h->bc_v.p, nbc, 1.0);
+  // prescribed values for the state; homogeneous constraints for a caller right-hand side
+  if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(u, h->bc_d.p, h->bc_v.p, nbc, b ? 0.0 : 1.0);
   PA_HIP(hipGetLastError());
   pa::MfArgs ma{};
   ma.chunk_rows = h->chunk_rows.p;
@@ -630,14 +634,21 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const pa_solver_
   ma.scal_out = pa::kPQ;
   ma.scal_num = pa::kRZ;
   ma.m = h->material();
-  const double bb = h->f_free_norm2;
+  double bb = h->f_free_norm2;
+  if (b) {  // ||b||^2 over free dofs, through the masked conversion of b - K 0
+    pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(h->m_R.p, ndof, 0.0);
+    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, b, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
+                                      h->scal.p, pa::kDot1, s));
+    if (int rc = read_scal(h)) return rc;
+    bb = h->host_scal[pa::kDot1];
+  }
   int inner_total = 0, outer = 0;
   double rel = 0.0;
   bool conv = false;
   const double itol = opts.inner_tolerance > 0 ? opts.inner_tolerance : 1e-4;
   while (true) {
-    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, true)) return rc;
-    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
+    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, b == nullptr)) return rc;
+    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, b, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
                                       h->scal.p, pa::kDot1, s));
     if (int rc = read_scal(h)) return rc;
     const double rr0 = h->host_scal[pa::kDot1];
@@ -687,7 +698,7 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   if (!z || !u) return fail(PA_ERR_ARG, "pa_solution: null pointer");
   pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
   if (opts.check_every <= 0) opts.check_every = 16;
-  if (opts.precision == 1) return solve_mixed(h, z, u, opts, st);
+  if (opts.precision == 1) return solve_mixed(h, z, u, nullptr, opts, st);
   if (opts.precision != 0) return fail(PA_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
   if (!vals) return fail(PA_ERR_ARG, "pa_solution: FP64 solve needs the values buffer");
   if (int rc = ensure_solver_ws(h)) return rc;
@@ -702,6 +713,94 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   if (int rc = pa_apply_constraints(h, vals, rhs, 1.0)) return rc;
   opts.use_x0 = 0;
   return pa_solve(h, vals, u, rhs, &opts, st);
+}
+
+// ---------------------------------------------------------------- stress p-norm + adjoint
+static int pnorm_value(pa_handle h, const double* u, const double* z, int measure, double pn, double q, double* S) {
+  pa::ChunkArgs a = h->chunk_args();
+  a.u = u;
+  a.z = z;
+  a.measure = measure;
+  a.pn = pn;
+  a.q = q;
+  PA_HIP(pa::launch_chunk(h->code_bits, pa::kModePNorm, a, h->stream));
+  return energy_value(h, S);
+}
+
+static int check_pnorm_args(int measure, double& pn, double q) {
+  if (measure != 0 && measure != 1) return fail(PA_ERR_ARG, "p-norm measure must be 0 (Voigt) or 1 (von Mises)");
+  if (measure == 0) pn = std::floor(pn);  // TensorPNormFunctor takes an integer exponent
+  if (!(pn >= 1.0)) return fail(PA_ERR_ARG, "p-norm exponent must be >= 1");
+  if (measure == 1 && !(q >= 0.0)) return fail(PA_ERR_ARG, "relaxation exponent q must be >= 0");
+  return PA_OK;
+}
+
+int pa_stress_pnorm(pa_handle h, const double* u, const double* z, int measure, double pn, double q, double* value,
+                    double* dfdu, double* dfdz) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !z) return fail(PA_ERR_ARG, "pa_stress_pnorm: null pointer");
+  if (int rc = check_pnorm_args(measure, pn, q)) return rc;
+  double S = 0.0;
+  if (int rc = pnorm_value(h, u, z, measure, pn, q, &S)) return rc;
+  if (value) *value = std::pow(S, 1.0 / pn);
+  if (!dfdu && !dfdz) return PA_OK;
+  if (int rc = ensure_solver_ws(h)) return rc;
+  pa::ChunkArgs a = h->chunk_args();
+  a.u = u;
+  a.z = z;
+  a.measure = measure;
+  a.pn = pn;
+  a.q = q;
+  a.pscale = S > 0.0 ? std::pow(S, (1.0 - pn) / pn) / pn : 0.0;
+  a.out_vec = dfdu ? dfdu : h->w_r.p;
+  a.out_vec2 = dfdz ? dfdz : h->w_z.p;  // w_z holds >= num_nodes doubles
+  PA_HIP(pa::launch_chunk(h->code_bits, pa::kModePNormGrad, a, h->stream));
+  return PA_OK;
+}
+
+int pa_dRdz_T_apply(pa_handle h, const double* u, const double* lambda, const double* z, double* out) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !lambda || !z || !out) return fail(PA_ERR_ARG, "pa_dRdz_T_apply: null pointer");
+  pa::ChunkArgs a = h->chunk_args();
+  a.u = u;
+  a.u2 = lambda;
+  a.z = z;
+  a.out_vec = out;
+  PA_HIP(pa::launch_chunk(h->code_bits, pa::kModeAdjZ, a, h->stream));
+  return PA_OK;
+}
+
+// Total derivative of the p-norm through K(z) u = f (Problem::criterionGradient, non
+// self-adjoint branch, src/elliptic/Problem.hpp:413-468): K lambda = dF/du with
+// homogeneous constraints, dF/dz = dF/dz|_u - (dR/dz)^T lambda.  lambda (3N, device)
+// receives the adjoint state.
+int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int measure, double pn, double q,
+                             double* dfdz, double* lambda, double* value, const pa_solver_opts* o,
+                             pa_solver_stats* st) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !z || !dfdz || !lambda) return fail(PA_ERR_ARG, "pa_stress_pnorm_gradient: null pointer");
+  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
+  if (opts.check_every <= 0) opts.check_every = 16;
+  if (int rc = ensure_solver_ws(h)) return rc;
+  const size_t ndof = 3 * (size_t)h->nn;
+  if (h->w_adj.n != ndof) PA_HIP(h->w_adj.alloc(ndof));
+  double* dfdu = h->w_adj.p;
+  if (int rc = pa_stress_pnorm(h, u, z, measure, pn, q, value, dfdu, dfdz)) return rc;
+  opts.use_x0 = 0;
+  if (opts.precision == 1) {
+    if (int rc = solve_mixed(h, z, lambda, dfdu, opts, st)) return rc;
+  } else {
+    const size_t nv = 9 * (size_t)h->graph.rowmap[h->nn];
+    if (h->w_K.n != nv) PA_HIP(h->w_K.alloc(nv));
+    if (int rc = run_chunk(h, pa::kModeJacobian, nullptr, z, h->w_K.p, nullptr)) return rc;
+    if (int rc = pa_apply_constraints(h, h->w_K.p, dfdu, 0.0)) return rc;  // homogeneous on lambda
+    if (int rc = pa_solve(h, h->w_K.p, lambda, dfdu, &opts, st)) return rc;
+  }
+  if (h->w_adjz.n != (size_t)h->nn) PA_HIP(h->w_adjz.alloc(h->nn));
+  if (int rc = pa_dRdz_T_apply(h, u, lambda, z, h->w_adjz.p)) return rc;
+  pa::k_axpy<<<grid_for((size_t)h->nn), 256, 0, h->stream>>>(dfdz, h->w_adjz.p, (size_t)h->nn, -1.0);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
 }
 
 int pa_compliance_gradient(pa_handle h, const double* u, const double* z, double* dcdz, double* value) {

@@ -103,8 +103,12 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   constexpr bool kRes = MODE == kModeResidual;
   constexpr bool kGzT = MODE == kModeGradZT;
   constexpr bool kGradZ = MODE == kModeEnergyGradZ;
-  constexpr int kBlkW = kGzT ? 3 : 0;                 // doubles per block in LDS
-  constexpr int kRowW = kRes ? 3 : (kGradZ ? 1 : 0);  // doubles per row in LDS
+  constexpr bool kPnV = MODE == kModePNorm, kPnG = MODE == kModePNormGrad;
+  constexpr bool kAdj = MODE == kModeAdjZ;
+  constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV;
+  constexpr int kBlkW = kGzT ? 3 : 0;                                  // doubles per block in LDS
+  constexpr int kRowW = (kRes || kPnG) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row in LDS
+  constexpr int kRow2 = kPnG ? 1 : 0;                                  // second per-row output
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
   const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1];
@@ -114,9 +118,11 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
   double4* nodes = reinterpret_cast<double4*>(lds);
   double* ul = reinterpret_cast<double*>(nodes + A.max_nodes);
-  double* seg = ul + 3 * (size_t)A.max_nodes;
+  double* ul2 = ul + 3 * (size_t)A.max_nodes;
+  double* seg = ul2 + (kAdj ? 3 * (size_t)A.max_nodes : 0);
   double* res = seg + (size_t)A.max_blocks * kBlkW;
-  int* roff = reinterpret_cast<int*>(res + (size_t)A.max_rows * kRowW);
+  double* res2 = res + (size_t)A.max_rows * kRowW;
+  int* roff = reinterpret_cast<int*>(res2 + (size_t)A.max_rows * kRow2);
 
   for (int k = threadIdx.x; k < nnode; k += NT) {
     const size_t gid = (size_t)A.nodes[n0 + k];
@@ -124,13 +130,18 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     ul[3 * k + 0] = A.u[3 * gid + 0];
     ul[3 * k + 1] = A.u[3 * gid + 1];
     ul[3 * k + 2] = A.u[3 * gid + 2];
+    if constexpr (kAdj) {
+      ul2[3 * k + 0] = A.u2[3 * gid + 0];
+      ul2[3 * k + 1] = A.u2[3 * gid + 1];
+      ul2[3 * k + 2] = A.u2[3 * gid + 2];
+    }
   }
   if constexpr (kBlkW > 0) {
     for (int k = threadIdx.x; k < nb * kBlkW; k += NT) seg[k] = 0.0;
     for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
   }
   if constexpr (kRowW > 0)
-    for (int k = threadIdx.x; k < nr * kRowW; k += NT) res[k] = 0.0;
+    for (int k = threadIdx.x; k < nr * (kRowW + kRow2); k += NT) res[k] = 0.0;
   __syncthreads();
 
   double energy = 0.0;
@@ -173,6 +184,74 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
         for (int a = 0; a < 4; ++a)
           if (ln[a] < nr) lds_add(&res[ln[a]], gz);
       }
+    }
+    if constexpr (kPnV || kPnG) {
+      // measure m and, for the gradient, the stress-space direction t with
+      // dm/du = c_u * B^T C0 t and dm/drho = c_z * m
+      double mval, t[6], cu, cz;
+      if (A.measure == 0) {
+        double ss = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) ss += (w * s[q]) * (w * s[q]);
+        mval = sqrt(ss);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) t[q] = w * s[q];  // d|sigma_w|/du = w B^T C0 sigma_w / m
+        cu = mval > 0.0 ? w / mval : 0.0;
+        cz = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1) / w;
+      } else {
+        const double d01 = s[0] - s[1], d12 = s[1] - s[2], d20 = s[2] - s[0];
+        const double j2 = 0.5 * (d01 * d01 + d12 * d12 + d20 * d20) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+        const double vm0 = sqrt(j2), rq = pow(rho, A.q);
+        mval = rq * vm0;
+        t[0] = 2.0 * s[0] - s[1] - s[2];
+        t[1] = 2.0 * s[1] - s[2] - s[0];
+        t[2] = 2.0 * s[2] - s[0] - s[1];
+        t[3] = 6.0 * s[3];
+        t[4] = 6.0 * s[4];
+        t[5] = 6.0 * s[5];
+        cu = vm0 > 0.0 ? rq / (2.0 * vm0) : 0.0;  // d vm / d sigma = dJ/dsigma / (2 vm)
+        cz = A.q / rho;
+      }
+      const double mp = pow(mval, A.pn);
+      if constexpr (kPnV) {
+        if (ln[0] < nr) energy += mp * vol;
+      } else {
+        // dS/du_a = vol p m^(p-1) c_u B_a^T C0 t ; dS/dz_a = vol p m^p c_z / 4
+        const double mpm1 = mval > 0.0 ? mp / mval : 0.0;
+        const double fu = A.pscale * vol * A.pn * mpm1 * cu;
+        const double fz = A.pscale * vol * A.pn * mp * cz * 0.25;
+        double tau[6];
+        iso_stress(A.m, t, tau);
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (ln[a] < nr) {
+            double r[3];
+            tet_divergence(tau, g[a], fu, r);
+            double* dst = &res[3 * ln[a]];
+            lds_add(dst + 0, r[0]);
+            lds_add(dst + 1, r[1]);
+            lds_add(dst + 2, r[2]);
+            lds_add(&res2[ln[a]], fz);
+          }
+        }
+      }
+    }
+    if constexpr (kAdj) {
+      // (dR/dz)^T lambda: dw/drho/4 * vol * eps(lambda_e) . sigma0(u_e)
+      double l[4][3], el[6];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) l[a][i] = ul2[3 * ln[a] + i];
+      tet_strain(l, g, el);
+      double dot = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) dot += el[q] * s[q];
+      const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
+      const double v = 0.25 * dw * vol * dot;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        if (ln[a] < nr) lds_add(&res[ln[a]], v);
     }
     if constexpr (kRes) {
       double sw[6];
@@ -225,8 +304,8 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     }
   }
 
-  if constexpr (MODE == kModeEnergy || kGradZ) {
-    // block reduction of the energy (fixed order -> deterministic)
+  if constexpr (kReduce) {
+    // block reduction of the energy / p-norm sum (fixed order -> deterministic)
     __shared__ double wsum[NT / 64];
     double v = energy;
 #pragma unroll
@@ -258,6 +337,10 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   if constexpr (kRowW > 0) {
     double* dst = A.out_vec + (size_t)r0 * kRowW;
     for (int k = threadIdx.x; k < nr * kRowW; k += NT) dst[k] = res[k];
+  }
+  if constexpr (kRow2 > 0) {
+    double* dst = A.out_vec2 + (size_t)r0;
+    for (int k = threadIdx.x; k < nr; k += NT) dst[k] = res2[k];
   }
 }
 
@@ -488,8 +571,12 @@ void k_jac_chunk(ChunkArgs A) {
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
   if (mode == kModeJacobian || mode == kModeJacobianResidual) return jac_lds_layout(max_rows, max_blocks, max_nodes).total;
   const int blkw = mode == kModeGradZT ? 3 : 0;
-  const int roww = mode == kModeResidual ? 3 : (mode == kModeEnergyGradZ ? 1 : 0);
-  size_t bytes = sizeof(double) * (7 * (size_t)max_nodes + (size_t)max_blocks * blkw + (size_t)max_rows * roww);
+  const int roww = (mode == kModeResidual || mode == kModePNormGrad) ? 3
+                   : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
+  const int row2 = mode == kModePNormGrad ? 1 : 0;
+  const int nodew = mode == kModeAdjZ ? 10 : 7;
+  size_t bytes = sizeof(double) * (nodew * (size_t)max_nodes + (size_t)max_blocks * blkw +
+                                   (size_t)max_rows * (roww + row2));
   if (blkw > 0) bytes += sizeof(int) * (size_t)(max_rows + 1);
   return bytes;
 }
@@ -510,6 +597,9 @@ static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
     case kModeGradZT: k_chunk<BITS, kModeGradZT, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeEnergy: k_chunk<BITS, kModeEnergy, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeEnergyGradZ: k_chunk<BITS, kModeEnergyGradZ, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModePNorm: k_chunk<BITS, kModePNorm, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModePNormGrad: k_chunk<BITS, kModePNormGrad, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeAdjZ: k_chunk<BITS, kModeAdjZ, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

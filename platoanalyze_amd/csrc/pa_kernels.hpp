@@ -32,6 +32,7 @@ struct ChunkArgs {
   const double* X;
   const double* z;
   const double* u;
+  const double* u2;   // second state (adjoint lambda) for kModeAdjZ
   const int32_t* rowmap;
   const int32_t* colidx;
   const int32_t* chunk_rows;
@@ -43,6 +44,7 @@ struct ChunkArgs {
   const uint32_t* ent_code;   // slot codes per entry
   double* out_vals;   // Jacobian values (9/block) or dR/dz (3/block)
   double* out_vec;    // residual (3/row) or nodal gradient (1/row)
+  double* out_vec2;   // p-norm dF/dz partial (1/row)
   double* partials;   // per-chunk partial sums (energy)
   // traction load of the chunk's rows, subtracted from the residual (R = K u - f)
   const int32_t* load_offs;  // [nchunks+1]
@@ -53,11 +55,16 @@ struct ChunkArgs {
   int max_rows;
   int max_blocks;
   int max_nodes;
+  // stress p-norm F = (sum vol m^p)^(1/p): measure 0 = Voigt norm of sigma_w (the
+  // reference StressPNorm), 1 = relaxed von Mises rho^q vm(sigma0); pscale multiplies
+  // the gradients (pow(S,(1-p)/p)/p, known after the value pass)
+  int measure;
+  double pn, q, pscale;
   Material m;
 };
 
 enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
-                 kModeEnergy = 4, kModeEnergyGradZ = 5 };
+                 kModeEnergy = 4, kModeEnergyGradZ = 5, kModePNorm = 6, kModePNormGrad = 7, kModeAdjZ = 8 };
 
 
 // launch helpers (defined in pa_kernels.hip)

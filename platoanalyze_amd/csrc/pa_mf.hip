@@ -314,8 +314,9 @@ __global__ __launch_bounds__(kT) void k_pcg32_pupdate(float* p, const float* z, 
   for (int d = blockIdx.x * kT + threadIdx.x; d < ndof; d += gridDim.x * kT) p[d] = z[d] + beta * p[d];
 }
 
-__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const uint8_t* bcmask, float* r32, int nn,
-                                                     double* partials, unsigned* counter, double* scal, int slot) {
+__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const double* b, const uint8_t* bcmask,
+                                                     float* r32, int nn, double* partials, unsigned* counter,
+                                                     double* scal, int slot) {
   __shared__ double sh[kT / 64];
   double rr = 0.0;
   for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
@@ -323,7 +324,7 @@ __global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const uint
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const size_t d = 3 * (size_t)n + i;
-      const double v = (bm >> i) & 1u ? 0.0 : -R[d];
+      const double v = (bm >> i) & 1u ? 0.0 : (b ? b[d] : 0.0) - R[d];
       r32[d] = (float)v;
       rr += v * v;
     }
@@ -383,9 +384,9 @@ hipError_t launch_pcg32_pupdate(float* p, const float* z, int ndof, const double
   k_pcg32_pupdate<<<grid_of(ndof), kT, 0, s>>>(p, z, ndof, scal);
   return hipGetLastError();
 }
-hipError_t launch_neg_mask_to_f32(const double* R, const uint8_t* bcmask, float* r32, int nn, double* partials,
-                                  unsigned* counter, double* scal, int slot, hipStream_t s) {
-  k_neg_mask_f32<<<grid_of(nn), kT, 0, s>>>(R, bcmask, r32, nn, partials, counter, scal, slot);
+hipError_t launch_neg_mask_to_f32(const double* R, const double* b, const uint8_t* bcmask, float* r32, int nn,
+                                  double* partials, unsigned* counter, double* scal, int slot, hipStream_t s) {
+  k_neg_mask_f32<<<grid_of(nn), kT, 0, s>>>(R, b, bcmask, r32, nn, partials, counter, scal, slot);
   return hipGetLastError();
 }
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nn, hipStream_t s) {

@@ -55,9 +55,10 @@ hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, con
                                int nnodes, double* partials, unsigned* counter, double* scal, hipStream_t s);
 hipError_t launch_pcg32_pupdate(float* p, const float* z, int ndof, const double* scal, hipStream_t s);
 
-// r32 = mask(-R64) (R = K u - f from the residual kernel), ||r||^2 into scal[slot]
-hipError_t launch_neg_mask_to_f32(const double* R, const uint8_t* bcmask, float* r32, int nnodes, double* partials,
-                                  unsigned* counter, double* scal, int slot, hipStream_t s);
+// r32 = mask(b - R) with b optional (R = K u - f from the residual kernel, or K x),
+// ||r||^2 into scal[slot]
+hipError_t launch_neg_mask_to_f32(const double* R, const double* b, const uint8_t* bcmask, float* r32, int nnodes,
+                                  double* partials, unsigned* counter, double* scal, int slot, hipStream_t s);
 // Xz[n] = (float) (X[3n], X[3n+1], X[3n+2], z[n])
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nnodes, hipStream_t s);
 // u += d (FP64 += FP32)

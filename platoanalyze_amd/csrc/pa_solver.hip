@@ -290,6 +290,10 @@ __global__ __launch_bounds__(kT) void k_fill(double* y, size_t n, double v) {
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] = v;
 }
 
+__global__ __launch_bounds__(kT) void k_axpy(double* y, const double* x, size_t n, double a) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] += a * x[i];
+}
+
 __global__ __launch_bounds__(kT) void k_scale(double* y, size_t n, double s) {
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] *= s;
 }

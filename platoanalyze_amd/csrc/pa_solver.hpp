@@ -25,6 +25,7 @@ __global__ void k_diag_inv(const double* vals, const int* diag_slot, double* din
 __global__ void k_add_load(double* y, const int* nodes, const double* f, int n, double s);
 __global__ void k_fill(double* y, size_t n, double v);
 __global__ void k_scale(double* y, size_t n, double s);
+__global__ void k_axpy(double* y, const double* x, size_t n, double a);
 __global__ void k_bc_lift(const double* vals, const int* rowmap, const int* col, const int* dofs,
                           const double* bcv, int nbc, double scale, double* rhs);
 __global__ void k_bc_zero(double* vals, const int* rowmap, const int* col, const int* dofs, int nbc);

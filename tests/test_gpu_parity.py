@@ -328,3 +328,66 @@ def test_gpu_mixed_precision_nonzero_dirichlet():
     assert mp.stats.converged
     np.testing.assert_allclose(uh[3 * fixed + 1], 0.01, rtol=0, atol=0)
     assert np.abs(uh - u64).max() <= 1e-7 * np.abs(u64).max()
+
+
+# ------------------------------------------------------------------ stress p-norm + adjoint
+def test_gpu_stress_pnorm_goldens_2x2x2(box222, goldens):
+    from platoanalyze_amd.criteria import StressPNorm
+
+    m = ElasticModel(box222, 1e6, 0.3, 1.0, 0.0)
+    u, z = dev(ref_state(27)), dev(np.ones(27))
+    g = goldens["stress_pnorm3d"]
+    sp = StressPNorm(m, 12.0, "voigt")
+    assert abs(sp.value(u, z) - g["value_gold"]) <= 1e-13 * g["value_gold"]
+    close(host(sp.gradient_u(u, z))[:27], g["grad_u_gold"], 1e-12, "pnorm grad_u")
+    close(host(sp.gradient_z(u, z)), g["grad_z_gold"], 1e-12, "pnorm grad_z")
+
+
+@pytest.mark.parametrize("measure,pn,q", [("voigt", 8.0, 0.0), ("von_mises", 8.0, 0.5), ("von_mises", 3.5, 1.0)])
+def test_gpu_stress_pnorm_partials_cfg0(cfg0, measure, pn, q):
+    from platoanalyze_amd.criteria import StressPNorm
+
+    w, m, o, rm, ci, u, z = cfg0
+    sp = StressPNorm(m, pn, measure, q)
+    du, dz = dev(u), dev(z)
+    F, gu, gz = o.pnorm(u, z, 0 if measure == "voigt" else 1, pn, q)
+    assert abs(sp.value(du, dz) - F) <= RTOL * abs(F)
+    close(host(sp.gradient_u(du, dz)), gu, RTOL, "dF/du")
+    close(host(sp.gradient_z(du, dz)), gz, RTOL, "dF/dz partial")
+
+
+def test_gpu_dRdz_T_apply_cfg0(cfg0):
+    from platoanalyze_amd.criteria import dRdz_T_apply
+
+    w, m, o, rm, ci, u, z = cfg0
+    lam = np.random.default_rng(31).normal(size=u.size)
+    close(host(dRdz_T_apply(m, dev(u), dev(lam), dev(z))), o.dRdz_T_apply(u, lam, z), RTOL, "(dR/dz)^T lambda")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gpu_pnorm_total_gradient_fd_and_autograd(precision):
+    """Hand-written adjoint of the relaxed von Mises p-norm (p=8, q=0.5) through a
+    torch.autograd.Function against central finite differences (h=1e-6) of full solves."""
+    from platoanalyze_amd import criteria
+
+    w = wl.make("cantilever_20x10x10", tiny=(8, 4, 4))
+    m = wl.model(w)
+    z0 = np.random.default_rng(13).uniform(0.3, 1.0, w.mesh.num_nodes)
+    tol = 1e-12 if precision == "fp64" else 1e-11
+    pb = Problem(m, tolerance=tol, precision=precision, inner_tolerance=1e-5)
+    z = dev(z0).requires_grad_(True)
+    F = criteria.stress_pnorm(z, pb, 8.0, "von_mises", 0.5)
+    F.backward()
+    g = host(z.grad)
+    C = criteria.compliance(z.detach(), pb)
+    rng = np.random.default_rng(64)
+    for n in rng.choice(w.mesh.num_nodes, 6, replace=False):
+        h = 1e-6
+        zp, zm = z0.copy(), z0.copy()
+        zp[n] += h
+        zm[n] -= h
+        fp = float(criteria.stress_pnorm(dev(zp), pb, 8.0, "von_mises", 0.5))
+        fm = float(criteria.stress_pnorm(dev(zm), pb, 8.0, "von_mises", 0.5))
+        fd = (fp - fm) / (2 * h)
+        assert abs(fd - g[n]) <= 1e-6 * np.abs(g).max() + 1e-7 * abs(fd), (n, fd, g[n])
+    assert float(C) > 0

@@ -125,9 +125,14 @@ def test_energy_golden(ref_model, goldens):
 
 
 def test_stress_pnorm_golden(ref_model, goldens):
-    """StressPNorm3D value (:1058), exponent 12."""
+    """StressPNorm3D value (:1058), exponent 12, and its partial gradients (:1069-1108)."""
+    g = goldens["stress_pnorm3d"]
     v = ref_model.stress_pnorm(ref_state(27), np.ones(27), 12.0)
-    assert abs(v - goldens["stress_pnorm3d"]["value_gold"]) <= 1e-13 * goldens["stress_pnorm3d"]["value_gold"]
+    assert abs(v - g["value_gold"]) <= 1e-13 * g["value_gold"]
+    v2, gu, gz = ref_model.pnorm(ref_state(27), np.ones(27), 0, 12.0)
+    assert v2 == v
+    assert_close(gu[:27], g["grad_u_gold"], 1e-12)
+    assert_close(gz, g["grad_z_gold"], 1e-12)
 
 
 # ----------------------------------------------------------------- invariants
@@ -258,3 +263,38 @@ def test_nonzero_dirichlet_lift(small):
     lhs = Kf[np.ix_(free, free)] @ x[free]
     rhs = b[free] - Kf[np.ix_(free, dofs)] @ vals
     np.testing.assert_allclose(lhs, rhs, atol=1e-10 * np.abs(rhs).max())
+
+
+def test_pnorm_von_mises_fd(small):
+    """Relaxed von Mises p-norm (no reference counterpart): partials against central
+    differences, and the homogeneity dF/dz = q F / (4 rho) per element for uniform rho."""
+    m, z, u, model, rm, ci = small
+    for measure, pn, q in [(1, 8.0, 0.5), (0, 8.0, 0.0)]:
+        F, gu, gz = model.pnorm(u, z, measure, pn, q)
+        d = np.random.default_rng(8).normal(size=u.size)
+        h = 1e-6
+        fd = (model.pnorm(u + h * d, z, measure, pn, q, False, False)[0]
+              - model.pnorm(u - h * d, z, measure, pn, q, False, False)[0]) / (2 * h)
+        assert abs(fd - gu @ d) <= 1e-6 * abs(fd)
+        for n in [0, 7, m.num_nodes // 2]:
+            zp, zm = z.copy(), z.copy()
+            zp[n] += h
+            zm[n] -= h
+            fdz = (model.pnorm(u, zp, measure, pn, q, False, False)[0]
+                   - model.pnorm(u, zm, measure, pn, q, False, False)[0]) / (2 * h)
+            assert abs(fdz - gz[n]) <= 1e-6 * np.abs(gz).max()
+    # uniform density: F(z) = rho^q F0  => sum_n dF/dz_n = q F / rho
+    zc = np.full_like(z, 0.4)
+    F, _, gz = model.pnorm(u, zc, 1, 8.0, 0.5, False, True)
+    np.testing.assert_allclose(gz.sum(), 0.5 * F / 0.4, rtol=1e-12)
+
+
+def test_dRdz_T_apply_matches_gradient_z(small):
+    m, z, u, model, rm, ci = small
+    lam = np.random.default_rng(12).normal(size=u.size)
+    G = model.gradient_z(u, z, rm, ci)  # rows = control node, entries dR_(col,i)/dz_row
+    want = np.zeros(m.num_nodes)
+    for r in range(m.num_nodes):
+        for s in range(rm[r], rm[r + 1]):
+            want[r] += G[3 * s:3 * s + 3] @ lam[3 * ci[s]:3 * ci[s] + 3]
+    np.testing.assert_allclose(model.dRdz_T_apply(u, lam, z), want, atol=1e-12 * np.abs(want).max())
