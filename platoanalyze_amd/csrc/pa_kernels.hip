@@ -141,7 +141,9 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
   }
   if constexpr (kRowW > 0)
-    for (int k = threadIdx.x; k < nr * (kRowW + kRow2); k += NT) res[k] = 0.0;
+    for (int k = threadIdx.x; k < nr * kRowW; k += NT) res[k] = 0.0;
+  if constexpr (kRow2 > 0)
+    for (int k = threadIdx.x; k < nr * kRow2; k += NT) res2[k] = 0.0;
   __syncthreads();
 
   double energy = 0.0;
