@@ -13,6 +13,7 @@
 #include "pa_host.hpp"
 #include "pa_kernels.hpp"
 #include "pa_mf.hpp"
+#include "pa_reduce.hpp"
 #include "pa_solver.hpp"
 
 namespace {
@@ -34,7 +35,6 @@ int fail(int code, const std::string& msg) {
 constexpr int kMaxChunkBlocks = 896;
 constexpr int kMaxChunkRows = 128;
 constexpr int kMaxChunkNodes = 256;  // node table entries (rows + coupled nodes) per chunk
-constexpr int kRedGrid = 1024;  // grid of the reduction kernels
 
 template <typename T>
 struct DevArray {
@@ -87,6 +87,7 @@ struct pa_problem {
   DevArray<int32_t> chunk_node_offs, chunk_nodes;
   DevArray<uint16_t> ent_ln;
   DevArray<uint32_t> ent_code;
+  DevArray<uint16_t> tslot;
   DevArray<int32_t> load_nodes, load_offs, load_row;
   DevArray<double> load_f;
   int nload = 0;
@@ -96,14 +97,13 @@ struct pa_problem {
   double f_free_norm2 = 0.0;
   // mixed-precision solver workspace
   DevArray<float4> m_xz;
-  DevArray<float> m_dinv, m_r, m_z, m_p, m_q, m_d;
+  DevArray<float> m_dinv, m_r, m_z, m_p, m_p2, m_q, m_d;
   DevArray<double> m_R;
-  DevArray<double> partials;
-  DevArray<unsigned> counter;
+  DevArray<double> partials;  // kNumPartialSlots x kRedBlocks
   DevArray<double> scal;
   DevArray<double> chunk_partials;
   // solver workspace (lazily allocated)
-  DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
+  DevArray<double> w_r, w_z, w_p, w_p2, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -131,6 +131,7 @@ struct pa_problem {
     a.nodes = chunk_nodes.p;
     a.ent_ln = ent_ln.p;
     a.ent_code = ent_code.p;
+    a.tslot = tslot.p;
     a.max_nodes = chunks.max_nodes;
     a.partials = chunk_partials.p;
     a.load_offs = load_offs.p;
@@ -357,6 +358,7 @@ int pa_finalize(pa_handle h) {
   PA_HIP(h->chunk_nodes.upload(h->chunks.nodes));
   PA_HIP(h->ent_ln.upload(h->chunks.ent_ln));
   PA_HIP(h->ent_code.upload(h->chunks.ent_code));
+  PA_HIP(h->tslot.upload(h->chunks.tslot));
   PA_HIP(h->load_nodes.upload(ln));
   PA_HIP(h->load_offs.upload(loffs));
   PA_HIP(h->load_row.upload(lrow));
@@ -373,9 +375,7 @@ int pa_finalize(pa_handle h) {
     h->f_free_norm2 = s2;
     PA_HIP(h->bcmask.upload(bm));
   }
-  PA_HIP(h->partials.alloc(3 * kRedGrid + 16));
-  PA_HIP(h->counter.alloc(4));
-  PA_HIP(hipMemset(h->counter.p, 0, 4 * sizeof(unsigned)));
+  PA_HIP(h->partials.alloc((size_t)pa::kNumPartialSlots * pa::kRedBlocks));
   PA_HIP(h->scal.alloc(pa::kNumScal));
   PA_HIP(hipMemset(h->scal.p, 0, pa::kNumScal * sizeof(double)));
   PA_HIP(h->chunk_partials.alloc(std::max(1, h->chunks.count)));
@@ -498,6 +498,7 @@ static int ensure_solver_ws(pa_handle h) {
     PA_HIP(h->w_r.alloc(ndof));
     PA_HIP(h->w_z.alloc(ndof));
     PA_HIP(h->w_p.alloc(ndof));
+    PA_HIP(h->w_p2.alloc(ndof));
     PA_HIP(h->w_q.alloc(ndof));
     PA_HIP(h->w_dinv.alloc(9 * (size_t)h->nn));
   }
@@ -521,32 +522,39 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
   hipStream_t s = h->stream;
-  const int gsp = grid_for((size_t)nn, 16, kRedGrid);
-  const int gv = grid_for((size_t)nn, 256, kRedGrid);
+  const int G = pa::red_grid((size_t)nn, 256);       // node-parallel kernels
+  const int Gs = pa::red_grid((size_t)nn, 16);       // row-group kernels (16 rows per group)
+  double* P = h->partials.p;
+  auto PS = [&](int slot) { return P + (size_t)slot * pa::kRedBlocks; };
   pa::k_diag_inv<<<grid_for(nn), 256, 0, s>>>(vals, h->diag_slot.p, h->w_dinv.p, nn);
   if (opts.use_x0) {
-    pa::k_spmv<<<grid_for((size_t)nn, 16, 65535), 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p,
-                                                             nn, 1);
+    pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p, nn, 1);
   } else {
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
     PA_HIP(hipMemcpyAsync(h->w_r.p, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  pa::k_pcg_init<<<gv, 256, 0, s>>>(h->w_r.p, h->w_z.p, h->w_p.p, b, h->w_dinv.p, nn, h->partials.p,
-                                     h->counter.p, h->scal.p);
+  pa::k_pcg_init<<<G, 256, 0, s>>>(h->w_r.p, h->w_z.p, b, h->w_dinv.p, nn, PS(pa::kPRZ0), PS(pa::kPRR),
+                                    PS(pa::kPBB));
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPBB), G, h->scal.p, pa::kBB);
   PA_HIP(hipGetLastError());
   if (int rc = read_scal(h)) return rc;
   const double bb = h->host_scal[pa::kBB];
   double rel = bb > 0 ? std::sqrt(h->host_scal[pa::kRR] / bb) : 0.0;
   int it = 0;
   bool conv = bb == 0.0 || rel <= opts.tolerance;
+  double* pbuf[2] = {h->w_p.p, h->w_p2.p};
   while (!conv && it < opts.max_iters) {
-    pa::k_pcg_spmv<<<gsp, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, h->w_p.p, h->w_q.p, nn, h->partials.p,
-                                       h->counter.p, h->scal.p);
-    pa::k_pcg_update<<<gv, 256, 0, s>>>(x, h->w_r.p, h->w_z.p, h->w_p.p, h->w_q.p, h->w_dinv.p, nn,
-                                        h->partials.p, h->counter.p, h->scal.p);
-    pa::k_pcg_pupdate<<<grid_for(ndof, 256, kRedGrid), 256, 0, s>>>(h->w_p.p, h->w_z.p, (int)ndof, h->scal.p);
+    const int cur = it & 1;
+    pa::k_pcg_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, h->w_z.p, pbuf[cur ^ 1], pbuf[cur],
+                                      h->w_q.p, nn, PS(pa::kPRZ0 + cur), PS(pa::kPRZ0 + (cur ^ 1)), G, it == 0,
+                                      PS(pa::kPPQ));
+    pa::k_pcg_update<<<G, 256, 0, s>>>(x, h->w_r.p, h->w_z.p, pbuf[cur], h->w_q.p, h->w_dinv.p, nn, PS(pa::kPPQ),
+                                       Gs, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRR),
+                                       h->scal.p);
     ++it;
     if (it % opts.check_every == 0 || it == opts.max_iters) {
+      pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
       PA_HIP(hipGetLastError());
       if (int rc = read_scal(h)) return rc;
       rel = std::sqrt(h->host_scal[pa::kRR] / bb);
@@ -555,10 +563,9 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
     }
   }
   // true residual
-  pa::k_spmv<<<grid_for((size_t)nn, 16, 65535), 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p,
-                                                           nn, 1);
-  pa::k_dot<<<grid_for(ndof, 256, kRedGrid), 256, 0, s>>>(h->w_r.p, h->w_r.p, ndof, h->partials.p,
-                                                         h->counter.p, h->scal.p, pa::kDot0);
+  pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p, nn, 1);
+  pa::k_dot<<<G, 256, 0, s>>>(h->w_r.p, h->w_r.p, ndof, PS(pa::kPDot));
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot0);
   PA_HIP(hipGetLastError());
   if (int rc = read_scal(h)) return rc;
   const double true_rel = bb > 0 ? std::sqrt(h->host_scal[pa::kDot0] / bb) : 0.0;
@@ -590,6 +597,7 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     PA_HIP(h->m_r.alloc(ndof));
     PA_HIP(h->m_z.alloc(ndof));
     PA_HIP(h->m_p.alloc(ndof));
+    PA_HIP(h->m_p2.alloc(ndof));
     PA_HIP(h->m_q.alloc(ndof));
     PA_HIP(h->m_d.alloc(ndof));
     PA_HIP(h->m_R.alloc(ndof));
@@ -615,6 +623,9 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   // prescribed values for the state; homogeneous constraints for a caller right-hand side
   if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(u, h->bc_d.p, h->bc_v.p, nbc, b ? 0.0 : 1.0);
   PA_HIP(hipGetLastError());
+  const int G = pa::red_grid((size_t)nn, 256);
+  double* P = h->partials.p;
+  auto PS = [&](int slot) { return P + (size_t)slot * pa::kRedBlocks; };
   pa::MfArgs ma{};
   ma.chunk_rows = h->chunk_rows.p;
   ma.node_offs = h->chunk_node_offs.p;
@@ -626,32 +637,35 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   ma.max_nodes = h->chunks.max_nodes;
   ma.Xz = h->m_xz.p;
   ma.bcmask = h->bcmask.p;
-  ma.x = h->m_p.p;
+  ma.z = h->m_z.p;
   ma.y = h->m_q.p;
-  ma.partials = h->chunk_partials.p;
-  ma.counter = h->counter.p;
-  ma.scal = h->scal.p;
-  ma.scal_out = pa::kPQ;
-  ma.scal_num = pa::kRZ;
+  ma.np = G;
+  ma.P_pq = PS(pa::kPPQ);
   ma.m = h->material();
+  const int Gm = pa::mf_apply_grid(h->chunks.count);
+  // ||r||^2 of the masked FP32 conversion of b - R into scal[slot]
+  auto masked_residual = [&](const double* R, double* out) -> int {
+    PA_HIP(pa::launch_neg_mask_to_f32(R, b, h->bcmask.p, h->m_r.p, nn, PS(pa::kPDot), G, s));
+    pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot1);
+    PA_HIP(hipGetLastError());
+    if (int rc = read_scal(h)) return rc;
+    *out = h->host_scal[pa::kDot1];
+    return PA_OK;
+  };
   double bb = h->f_free_norm2;
   if (b) {  // ||b||^2 over free dofs, through the masked conversion of b - K 0
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(h->m_R.p, ndof, 0.0);
-    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, b, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
-                                      h->scal.p, pa::kDot1, s));
-    if (int rc = read_scal(h)) return rc;
-    bb = h->host_scal[pa::kDot1];
+    if (int rc = masked_residual(h->m_R.p, &bb)) return rc;
   }
   int inner_total = 0, outer = 0;
   double rel = 0.0;
   bool conv = false;
   const double itol = opts.inner_tolerance > 0 ? opts.inner_tolerance : 1e-4;
+  float* pbuf[2] = {h->m_p.p, h->m_p2.p};
   while (true) {
     if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, b == nullptr)) return rc;
-    PA_HIP(pa::launch_neg_mask_to_f32(h->m_R.p, b, h->bcmask.p, h->m_r.p, nn, h->partials.p, h->counter.p,
-                                      h->scal.p, pa::kDot1, s));
-    if (int rc = read_scal(h)) return rc;
-    const double rr0 = h->host_scal[pa::kDot1];
+    double rr0 = 0.0;
+    if (int rc = masked_residual(h->m_R.p, &rr0)) return rc;
     rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
     if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve: non-finite residual");
     if (rel <= opts.tolerance || bb == 0.0) {
@@ -661,18 +675,25 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     if (inner_total >= opts.max_iters || outer >= 50) break;
     ++outer;
     // inner FP32 PCG on K d = r from d = 0
-    PA_HIP(hipMemsetAsync(h->m_d.p, 0, ndof * sizeof(float), s));
-    PA_HIP(pa::launch_pcg32_init(h->m_r.p, h->m_z.p, h->m_p.p, h->m_dinv.p, nn, h->partials.p, h->counter.p,
-                                 h->scal.p, s));
+    PA_HIP(pa::launch_zero_f32(h->m_d.p, ndof, s));
+    PA_HIP(pa::launch_pcg32_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
     int it = 0;
     while (inner_total < opts.max_iters) {
+      const int cur = it & 1;
+      ma.first = it == 0;
+      ma.p_prev = pbuf[cur ^ 1];
+      ma.p_cur = pbuf[cur];
+      ma.P_rz_cur = PS(pa::kPRZ0 + cur);
+      ma.P_rz_prev = PS(pa::kPRZ0 + (cur ^ 1));
       PA_HIP(pa::launch_mf_apply(ma, s));
-      PA_HIP(pa::launch_pcg32_update(h->m_d.p, h->m_r.p, h->m_z.p, h->m_p.p, h->m_q.p, h->m_dinv.p, nn,
-                                     h->partials.p, h->counter.p, h->scal.p, s));
-      PA_HIP(pa::launch_pcg32_pupdate(h->m_p.p, h->m_z.p, (int)ndof, h->scal.p, s));
+      PA_HIP(pa::launch_pcg32_update(h->m_d.p, h->m_r.p, h->m_z.p, pbuf[cur], h->m_q.p, h->m_dinv.p, nn,
+                                     PS(pa::kPPQ), Gm, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)),
+                                     PS(pa::kPRR), h->scal.p, G, s));
       ++it;
       ++inner_total;
       if (it % opts.check_every == 0) {
+        pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
+        PA_HIP(hipGetLastError());
         if (int rc = read_scal(h)) return rc;
         const double ir = std::sqrt(h->host_scal[pa::kRR] / rr0);
         if (!std::isfinite(ir)) return fail(PA_ERR_NOT_CONVERGED, "inner FP32 PCG: non-finite residual");

@@ -429,6 +429,19 @@ Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, c
       }
     }
   });
+  ch.tslot.assign(g.colidx.size(), kNoSlot);
+  parallel_for(ch.count, [&](int64_t b, int64_t e) {
+    for (int64_t c = b; c < e; ++c) {
+      const int r0 = ch.row_begin[c], r1 = ch.row_begin[c + 1], b0 = g.rowmap[r0];
+      for (int r = r0; r < r1; ++r)
+        for (int k = g.rowmap[r]; k < g.rowmap[r + 1]; ++k) {
+          const int cc = g.colidx[k];
+          if (cc < r0 || cc >= r) continue;
+          for (int t = g.rowmap[cc]; t < g.rowmap[cc + 1]; ++t)
+            if (g.colidx[t] == r) ch.tslot[k] = (uint16_t)(t - b0);
+        }
+    }
+  });
   return ch;
 }
 

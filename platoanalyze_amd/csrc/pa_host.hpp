@@ -68,6 +68,8 @@ std::vector<uint32_t> slot_codes(const Graph& g, const int32_t* conn, int ne, in
 //       ent_ln   the element's 4 vertices as local node-table indices (uint16),
 //       ent_code the element's slot codes (words_per_code uint32 each),
 //     sorted by owned-vertex count and interleaved (see build_chunks).
+constexpr uint16_t kNoSlot = 0xFFFF;
+
 struct Chunks {
   int count = 0;
   std::vector<int32_t> row_begin;    // [count+1]
@@ -77,6 +79,9 @@ struct Chunks {
   std::vector<int32_t> nodes;        // node tables
   std::vector<uint16_t> ent_ln;      // 4 per entry
   std::vector<uint32_t> ent_code;    // words_per_code per entry
+  // per graph block (row r, column c): chunk-local slot of block (c, r) when both rows
+  // lie in the chunk and c < r, else kNoSlot
+  std::vector<uint16_t> tslot;
   int max_rows = 0, max_blocks = 0, max_nodes = 0;
 };
 Chunks build_chunks(const Graph& g, const NodeElems& ne_adj, const int32_t* conn, const std::vector<uint32_t>& codes,

@@ -6,6 +6,8 @@
 
 namespace pa {
 
+constexpr int kNoSlotDev = 0xFFFF;  // no transposed partner in the chunk
+
 // Work-groups are dispatched round-robin over the 8 XCDs; give each XCD one
 // contiguous run of chunks so neighbouring chunks (which share elements and nodes)
 // hit the same L2.
@@ -346,9 +348,45 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   }
 }
 
-// Local edges (a,b), a<b, of a tet packed 4 bits per edge index t = 0..5.
-constexpr uint32_t kEdgeA = 0x211000u;  // 0,0,0,1,1,2
-constexpr uint32_t kEdgeB = 0x332321u;  // 1,2,3,2,3,3
+// Local edges (a,b), a<b, of a tet.
+__device__ constexpr int kEdgeA[6] = {0, 0, 0, 1, 1, 2};
+__device__ constexpr int kEdgeB[6] = {1, 2, 3, 2, 3, 3};
+
+// Cross-lane moves inside an aligned group of 16 lanes (one DPP row), without LDS.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+// DPP controls: quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror, row_mirror
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppHalfMirror = 0x141, kDppMirror = 0x140;
+
+// Sum over each aligned 16-lane group; every lane of the group receives the same bits.
+// After the two quad steps each lane holds its quad's sum; the half-mirror pairs each
+// lane with one of the other quad of its half, the mirror with one of the other half.
+__device__ __forceinline__ double sum16(double v) {
+  v += dpp_f64<kDppXor1>(v);
+  v += dpp_f64<kDppXor2>(v);
+  v += dpp_f64<kDppHalfMirror>(v);
+  v += dpp_f64<kDppMirror>(v);
+  return v;
+}
+__device__ __forceinline__ int max16(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppXor1, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppXor2, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppHalfMirror, 0xF, 0xF, false));
+  v = max(v, __builtin_amdgcn_update_dpp(0, v, kDppMirror, 0xF, 0xF, false));
+  return v;
+}
+
+// Slot code of block (row vertex x, column vertex y) of an element entry.
+template <int BITS>
+__device__ __forceinline__ uint32_t slot_code(const uint32_t cw[4], int x, int y) {
+  if constexpr (BITS == 4) return (cw[x >> 1] >> (16 * (x & 1) + 4 * y)) & 15u;
+  else return (cw[x] >> (8 * y)) & 255u;
+}
 
 // LDS layout of the Jacobian kernel (byte offsets, 16-byte aligned sections).
 struct JacLds {
@@ -373,12 +411,14 @@ __host__ __device__ inline JacLds jac_lds_layout(int max_rows, int max_blocks, i
 //     gathers.  A vertex is owned by the chunk iff its local index < number of rows.
 //   * Off-diagonal blocks are accumulated once per element edge: in the row of the
 //     smaller vertex number when both ends are owned (the other block is its
-//     transpose, K being symmetric), else in the row of the owned end.
+//     transpose, K being symmetric), else in the row of the owned end.  The six
+//     edges are unrolled, so vertex pairs are compile-time constants.
 //   * Diagonal blocks use the rigid-translation identity of every element,
 //     sum_b K_ab = 0 (sum_b grad N_b = 0): K_rr = -sum_{c != r} K_rc.
 //   * Residual R_r = sum_c K_rc u_c - f_r = sum_{c != r} K_rc (u_c - u_r) - f_r.
-// Transposes, diagonals and residual are finished by 16-lane row groups with
-// cross-lane reductions, then the chunk's CRS segment is streamed out once.
+// Transposes (slot of the partner block from a setup table), diagonals and residual
+// are finished by 16-lane row groups with DPP reductions, then the chunk's CRS
+// segment is streamed out once.
 template <int BITS, bool WITH_R, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
 void k_jac_chunk(ChunkArgs A) {
@@ -398,7 +438,12 @@ void k_jac_chunk(ChunkArgs A) {
   const int nb = A.rowmap[r1] - b0;
   const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
 
-  for (int k = threadIdx.x; k < nb * 9; k += NT) seg[k] = 0.0;
+  {
+    // seg starts 16-byte aligned and holds 9*nb doubles
+    double2* s2 = reinterpret_cast<double2*>(seg);
+    const int n2 = (9 * nb + 1) / 2;
+    for (int k = threadIdx.x; k < n2; k += NT) s2[k] = make_double2(0.0, 0.0);
+  }
   for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
   for (int k = threadIdx.x; k < nb; k += NT) col[k] = A.colidx[b0 + k];
   for (int k = threadIdx.x; k < nnode; k += NT) {
@@ -449,44 +494,33 @@ void k_jac_chunk(ChunkArgs A) {
     rho = rho / 4.0;
     const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
     const double vl = vol * w * A.m.c01, vm = vol * w * A.m.c33;
-    uint32_t mask = 0, emask = 0;
+    bool own[4];
+    int ro[4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) mask |= (ln[a] < nr ? 1u : 0u) << a;
+    for (int a = 0; a < 4; ++a) {
+      own[a] = ln[a] < nr;
+      ro[a] = own[a] ? roff[ln[a]] : 0;
+    }
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-      const int a = (kEdgeA >> (4 * t)) & 15, b = (kEdgeB >> (4 * t)) & 15;
-      emask |= (((mask >> a) | (mask >> b)) & 1u) << t;
-    }
-    // lists are sorted by owned-vertex count -> near-uniform trip counts per wave
-    for (uint32_t bits = emask; bits; bits &= bits - 1) {
-      const int t = __builtin_ctz(bits);
-      const int a = (kEdgeA >> (4 * t)) & 15, b = (kEdgeB >> (4 * t)) & 15;
-      const int la = sel4i(a, ln[0], ln[1], ln[2], ln[3]), lb = sel4i(b, ln[0], ln[1], ln[2], ln[3]);
-      const bool oa = la < nr, ob = lb < nr;
-      // both owned: the row of the smaller vertex (local order = global order for rows)
-      const bool xa_is_a = oa && (!ob || la < lb);
-      const int xa = xa_is_a ? a : b, ya = xa_is_a ? b : a;
-      const int lx = xa_is_a ? la : lb;
-      double gx[3], gy[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        gx[i] = sel4(xa, g[0][i], g[1][i], g[2][i], g[3][i]);
-        gy[i] = sel4(ya, g[0][i], g[1][i], g[2][i], g[3][i]);
-      }
-      uint32_t code;
-      if constexpr (BITS == 4) code = ((xa < 2 ? cw[0] : cw[1]) >> (16 * (xa & 1) + 4 * ya)) & 15u;
-      else code = ((xa == 0 ? cw[0] : (xa == 1 ? cw[1] : (xa == 2 ? cw[2] : cw[3]))) >> (8 * ya)) & 255u;
-      double* dst = &seg[9 * (roff[lx] + (int)code)];
-      // K_xy = vw (lam g_x g_y^T + mu g_y g_x^T + mu (g_x.g_y) I)
-      const double d = vm * (gx[0] * gy[0] + gx[1] * gy[1] + gx[2] * gy[2]);
+      const int a = kEdgeA[t], b = kEdgeB[t];
+      if (!(own[a] || own[b])) continue;
+      // row b when a is not owned, or both are and b has the smaller vertex number
+      // (local order = global order for rows)
+      const bool rowb = !own[a] || (own[b] && ln[b] < ln[a]);
+      const int slot = rowb ? ro[b] + (int)slot_code<BITS>(cw, b, a) : ro[a] + (int)slot_code<BITS>(cw, a, b);
+      double* dst = &seg[9 * slot];
+      // K_ab = vw (lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I); row b stores K_ab^T
+      const double d = vm * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
+      double kv[3][3];
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          double v = vl * gx[i] * gy[j] + vm * gy[i] * gx[j];
-          if (i == j) v += d;
-          lds_add(dst + 3 * i + j, v);
-        }
+        for (int j = 0; j < 3; ++j) kv[i][j] = (vl * g[a][i]) * g[b][j] + (vm * g[b][i]) * g[a][j] + (i == j ? d : 0.0);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) lds_add(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j]);
     }
   }
   __syncthreads();
@@ -494,6 +528,7 @@ void k_jac_chunk(ChunkArgs A) {
   // Row groups of 16 lanes finish one row each: transposed blocks, diagonal, residual.
   constexpr int kG = 16;
   const int lane = threadIdx.x % kG;
+  const uint16_t* tsl = A.tslot + b0;
   for (int lr = threadIdx.x / kG; lr < nr; lr += NT / kG) {
     const int rr = r0 + lr;
     double sum[9];
@@ -513,21 +548,14 @@ void k_jac_chunk(ChunkArgs A) {
         continue;
       }
       double blk[9];
-      if (cc >= r0 && cc < rr) {  // both rows in the chunk: transpose of block (cc, rr)
-        int p = -1;
-        for (int q = roff[cc - r0]; q < roff[cc - r0 + 1]; ++q)
-          if (col[q] == rr) p = q;
-        if (p >= 0) {
+      const int p = tsl[s2];
+      if (p != kNoSlotDev) {  // both rows in the chunk, below the diagonal: transpose of (cc, rr)
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) blk[3 * i + j] = seg[9 * p + 3 * j + i];
+          for (int j = 0; j < 3; ++j) blk[3 * i + j] = seg[9 * p + 3 * j + i];
 #pragma unroll
-          for (int q = 0; q < 9; ++q) seg[9 * s2 + q] = blk[q];
-        } else {  // unsymmetric caller graph: keep what was accumulated
-#pragma unroll
-          for (int q = 0; q < 9; ++q) blk[q] = seg[9 * s2 + q];
-        }
+        for (int q = 0; q < 9; ++q) seg[9 * s2 + q] = blk[q];
       } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) blk[q] = seg[9 * s2 + q];
@@ -542,15 +570,12 @@ void k_jac_chunk(ChunkArgs A) {
       }
     }
 #pragma unroll
-    for (int off = kG / 2; off > 0; off >>= 1) {
+    for (int q = 0; q < 9; ++q) sum[q] = sum16(sum[q]);
+    if constexpr (WITH_R) {
 #pragma unroll
-      for (int q = 0; q < 9; ++q) sum[q] += __shfl_xor(sum[q], off, kG);
-      if constexpr (WITH_R) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) ru[i] += __shfl_xor(ru[i], off, kG);
-      }
-      dslot = max(dslot, __shfl_xor(dslot, off, kG));
+      for (int i = 0; i < 3; ++i) ru[i] = sum16(ru[i]);
     }
+    dslot = max16(dslot);
     if (lane < 9 && dslot >= 0) {
       double v = sum[0];
 #pragma unroll
@@ -566,8 +591,17 @@ void k_jac_chunk(ChunkArgs A) {
     }
   }
   __syncthreads();
+  // stream the segment out, 16 bytes per store where the destination allows
   double* dst = A.out_vals + (size_t)b0 * 9;
-  for (int k = threadIdx.x; k < nb * 9; k += NT) dst[k] = seg[k];
+  const int n = 9 * nb;
+  const int head = (int)(((uintptr_t)dst >> 3) & 1);  // one double before a 16-byte boundary
+  if (head && threadIdx.x == 0 && n > 0) dst[0] = seg[0];
+  const int n2 = (n - head) / 2;
+  for (int k = threadIdx.x; k < n2; k += NT) {
+    const int e = head + 2 * k;
+    *reinterpret_cast<double2*>(dst + e) = make_double2(seg[e], seg[e + 1]);
+  }
+  if (((n - head) & 1) && threadIdx.x == 0) dst[n - 1] = seg[n - 1];
 }
 
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {

@@ -42,6 +42,7 @@ struct ChunkArgs {
   const int32_t* nodes;       // node tables (chunk rows first)
   const uint16_t* ent_ln;     // 4 local node indices per entry
   const uint32_t* ent_code;   // slot codes per entry
+  const uint16_t* tslot;      // per graph block: chunk-local slot of the transposed block
   double* out_vals;   // Jacobian values (9/block) or dR/dz (3/block)
   double* out_vec;    // residual (3/row) or nodal gradient (1/row)
   double* out_vec2;   // p-norm dF/dz partial (1/row)

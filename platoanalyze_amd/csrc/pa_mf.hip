@@ -1,6 +1,7 @@
 // Matrix-free FP32 operator, FP64 block-diagonal preconditioner and FP32 PCG steps
 // for the mixed-precision solve (inner FP32 PCG, outer FP64 iterative refinement).
 #include "pa_mf.hpp"
+#include "pa_reduce.hpp"
 #include "pa_solver.hpp"
 
 namespace pa {
@@ -13,28 +14,6 @@ __device__ __forceinline__ int xcd_map(int b, int n) {
   const int q = n / kNumXcd, r = n % kNumXcd;
   const int x = b % kNumXcd, k = b / kNumXcd;
   return x * q + (x < r ? x : r) + k;
-}
-
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < (int)(blockDim.x / 64); ++q) t += sh[q];
-  return t;
-}
-
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ int am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) am_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last != 0;
 }
 
 __device__ __forceinline__ float powi_f(float x, float p, int pint) {
@@ -68,8 +47,12 @@ __device__ __forceinline__ void gradients_t(const T x[4][3], T g[4][3], T& vol) 
   vol = fabs(det) * (T(1) / T(6));
 }
 
-// y = K x (FP32): chunk node table with FP32 coordinates/density and the input
-// vector staged in LDS; residual-style element loop; ds_add_f32 into the owned rows.
+// q = K p (FP32) with the PCG direction update fused in.  Persistent work-groups walk
+// the chunks of their XCD's contiguous share.  Per chunk: the node table (FP32
+// coordinates + density) and p = z + beta p_prev at the table's nodes are staged in
+// LDS (constrained dofs read as 0), the chunk's own rows of p are written out, the
+// element loop accumulates K_e p_e into the owned rows with ds_add_f32, and the rows
+// are streamed out with the p.q partial.
 template <int NT>
 __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
   extern __shared__ float smem[];
@@ -78,84 +61,89 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
   float* acc = xl + 3 * A.max_nodes;
   __shared__ double sh[NT / 64];
 
-  const int c = xcd_map(blockIdx.x, A.nchunks);
-  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
-  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
-  for (int k = threadIdx.x; k < nnode; k += NT) {
-    const int gid = A.nodes[n0 + k];
-    nd4[k] = A.Xz[gid];
-    const uint32_t bm = A.bcmask[gid];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) xl[3 * k + i] = (bm >> i) & 1u ? 0.0f : A.x[3 * (size_t)gid + i];
+  float beta = 0.0f;
+  if (!A.first) {
+    const double a = sum_partials_all<NT>(A.P_rz_cur, A.np, sh);
+    const double b = sum_partials_all<NT>(A.P_rz_prev, A.np, sh);
+    beta = (float)(a / b);
   }
-  for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0f;
-  __syncthreads();
-
   const float c00 = (float)A.m.c00, c01 = (float)A.m.c01, c33 = (float)A.m.c33;
   const float rmin = (float)A.m.rmin, pp = (float)A.m.p;
-  const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
-  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
-    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
-    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
-    float x[4][3], g[4][3], u[4][3], vol, rho = 0.0f;
+  double dot = 0.0;
+  const XcdSpan sp = xcd_span(A.nchunks);
+  for (int c = sp.lo + sp.j; c < sp.hi; c += sp.bx) {
+    const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
+    const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+    for (int k = threadIdx.x; k < nnode; k += NT) {
+      const int gid = A.nodes[n0 + k];
+      nd4[k] = A.Xz[gid];
+      const uint32_t bm = A.bcmask[gid];
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const float4 p = nd4[ln[a]];
-      x[a][0] = p.x;
-      x[a][1] = p.y;
-      x[a][2] = p.z;
-      rho += p.w;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) u[a][i] = xl[3 * ln[a] + i];
-    }
-    gradients_t(x, g, vol);
-    rho *= 0.25f;
-    const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
-    float e[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      e[0] += u[a][0] * g[a][0];
-      e[1] += u[a][1] * g[a][1];
-      e[2] += u[a][2] * g[a][2];
-      e[3] += u[a][2] * g[a][1] + u[a][1] * g[a][2];
-      e[4] += u[a][2] * g[a][0] + u[a][0] * g[a][2];
-      e[5] += u[a][1] * g[a][0] + u[a][0] * g[a][1];
-    }
-    const float s0 = vw * (c00 * e[0] + c01 * e[1] + c01 * e[2]);
-    const float s1 = vw * (c01 * e[0] + c00 * e[1] + c01 * e[2]);
-    const float s2 = vw * (c01 * e[0] + c01 * e[1] + c00 * e[2]);
-    const float s3 = vw * c33 * e[3], s4 = vw * c33 * e[4], s5 = vw * c33 * e[5];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      if (ln[a] < nr) {
-        float* dst = &acc[3 * ln[a]];
-        atomicAdd(dst + 0, s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]);
-        atomicAdd(dst + 1, s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]);
-        atomicAdd(dst + 2, s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]);
+      for (int i = 0; i < 3; ++i) {
+        const size_t d = 3 * (size_t)gid + i;
+        float v = A.first ? A.z[d] : fmaf(beta, A.p_prev[d], A.z[d]);
+        v = (bm >> i) & 1u ? 0.0f : v;
+        xl[3 * k + i] = v;
+        if (k < nr) A.p_cur[d] = v;  // rows come first in the table
       }
     }
-  }
-  __syncthreads();
-  double dot = 0.0;
-  for (int k = threadIdx.x; k < 3 * nr; k += NT) {
-    const size_t gd = 3 * (size_t)r0 + k;
-    const float xv = A.x[gd];
-    const float yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? xv : acc[k];
-    A.y[gd] = yv;
-    dot += (double)xv * (double)yv;
-  }
-  const double t = block_sum(dot, sh);
-  if (threadIdx.x == 0) A.partials[c] = t;
-  if (last_block(A.counter)) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < A.nchunks; i += NT) v += A.partials[i];
-    const double tot = block_sum(v, sh);
-    if (threadIdx.x == 0) {
-      if (A.scal_out >= 0) A.scal[A.scal_out] = tot;
-      if (A.scal_num >= 0) A.scal[kAlpha] = A.scal[A.scal_num] / tot;
-      *A.counter = 0;
+    for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0f;
+    __syncthreads();
+
+    const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+    for (int k = l0 + threadIdx.x; k < l1; k += NT) {
+      const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+      const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+      float x[4][3], g[4][3], u[4][3], vol, rho = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const float4 p = nd4[ln[a]];
+        x[a][0] = p.x;
+        x[a][1] = p.y;
+        x[a][2] = p.z;
+        rho += p.w;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) u[a][i] = xl[3 * ln[a] + i];
+      }
+      gradients_t(x, g, vol);
+      rho *= 0.25f;
+      const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
+      float e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        e[0] += u[a][0] * g[a][0];
+        e[1] += u[a][1] * g[a][1];
+        e[2] += u[a][2] * g[a][2];
+        e[3] += u[a][2] * g[a][1] + u[a][1] * g[a][2];
+        e[4] += u[a][2] * g[a][0] + u[a][0] * g[a][2];
+        e[5] += u[a][1] * g[a][0] + u[a][0] * g[a][1];
+      }
+      const float s0 = vw * (c00 * e[0] + c01 * e[1] + c01 * e[2]);
+      const float s1 = vw * (c01 * e[0] + c00 * e[1] + c01 * e[2]);
+      const float s2 = vw * (c01 * e[0] + c01 * e[1] + c00 * e[2]);
+      const float s3 = vw * c33 * e[3], s4 = vw * c33 * e[4], s5 = vw * c33 * e[5];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (ln[a] < nr) {
+          float* dst = &acc[3 * ln[a]];
+          atomicAdd(dst + 0, s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]);
+          atomicAdd(dst + 1, s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]);
+          atomicAdd(dst + 2, s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]);
+        }
+      }
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < 3 * nr; k += NT) {
+      const size_t gd = 3 * (size_t)r0 + k;
+      const float xv = xl[k];
+      const float yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? xv : acc[k];
+      A.y[gd] = yv;
+      dot += (double)xv * (double)yv;
+    }
+    __syncthreads();  // the next chunk reuses the LDS tables
   }
+  const double t = block_reduce_all<NT>(dot, sh);
+  if (threadIdx.x == 0) A.P_pq[blockIdx.x] = t;
 }
 
 // Diagonal blocks K_rr (FP64) accumulated per owned vertex, then inverted per row.
@@ -231,95 +219,82 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
   }
 }
 
-__device__ __forceinline__ void dinv_apply(const float* dinv, const float* r, float* z, int n) {
-  const float* D = dinv + 9 * (size_t)n;
-  const float a = r[3 * (size_t)n], b = r[3 * (size_t)n + 1], cc = r[3 * (size_t)n + 2];
-  z[3 * (size_t)n + 0] = D[0] * a + D[1] * b + D[2] * cc;
-  z[3 * (size_t)n + 1] = D[3] * a + D[4] * b + D[5] * cc;
-  z[3 * (size_t)n + 2] = D[6] * a + D[7] * b + D[8] * cc;
+__device__ __forceinline__ void dinv3f(const float* __restrict__ D, const float r[3], float z[3]) {
+  z[0] = D[0] * r[0] + D[1] * r[1] + D[2] * r[2];
+  z[1] = D[3] * r[0] + D[4] * r[1] + D[5] * r[2];
+  z[2] = D[6] * r[0] + D[7] * r[1] + D[8] * r[2];
 }
 
-__global__ __launch_bounds__(kT) void k_pcg32_init(const float* r, float* z, float* p, const float* dinv, int nn,
-                                                   double* partials, unsigned* counter, double* scal) {
+__global__ __launch_bounds__(kT) void k_pcg32_init(const float* __restrict__ r, float* __restrict__ z,
+                                                   const float* __restrict__ dinv, int nn,
+                                                   double* __restrict__ P_rz) {
   __shared__ double sh[kT / 64];
   double rz = 0.0;
-  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
-    dinv_apply(dinv, r, z, n);
+  const XcdSpan sp = xcd_span(nn);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
+    const size_t d = 3 * (size_t)n;
+    const float rv[3] = {r[d], r[d + 1], r[d + 2]};
+    float zv[3];
+    dinv3f(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      p[d] = z[d];
-      rz += (double)r[d] * (double)z[d];
+      z[d + i] = zv[i];
+      rz += (double)rv[i] * (double)zv[i];
     }
   }
-  const double t = block_sum(rz, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-  if (last_block(counter)) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) v += partials[i];
-    const double tot = block_sum(v, sh);
-    if (threadIdx.x == 0) {
-      scal[kRZ] = tot;
-      *counter = 0;
-    }
-  }
+  const double t = block_reduce_all<kT>(rz, sh);
+  if (threadIdx.x == 0) P_rz[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kT) void k_pcg32_update(float* x, float* r, float* z, const float* p, const float* q,
-                                                     const float* dinv, int nn, double* partials, unsigned* counter,
-                                                     double* scal) {
+__global__ __launch_bounds__(kT) void k_pcg32_update(float* __restrict__ x, float* __restrict__ r,
+                                                     float* __restrict__ z, const float* __restrict__ p,
+                                                     const float* __restrict__ q, const float* __restrict__ dinv,
+                                                     int nn, const double* P_pq, int np_pq, const double* P_rz,
+                                                     int np, double* __restrict__ P_rz_next,
+                                                     double* __restrict__ P_rr, double* __restrict__ scal) {
   __shared__ double sh[kT / 64];
-  const float alpha = (float)scal[kAlpha];
-  double rz = 0.0, rr = 0.0;
-  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+  const double pq = sum_partials_all<kT>(P_pq, np_pq, sh);
+  const double rz = sum_partials_all<kT>(P_rz, np, sh);
+  const float alpha = (float)(rz / pq);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[kPQ] = pq;
+    scal[kRZ] = rz;
+    scal[kAlpha] = alpha;
+  }
+  double rzn = 0.0, rr = 0.0;
+  const XcdSpan sp = xcd_span(nn);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
+    const size_t d = 3 * (size_t)n;
+    float rv[3], zv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      x[d] += alpha * p[d];
-      r[d] -= alpha * q[d];
+      x[d + i] += alpha * p[d + i];
+      rv[i] = r[d + i] - alpha * q[d + i];
+      r[d + i] = rv[i];
     }
-    dinv_apply(dinv, r, z, n);
+    dinv3f(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      rz += (double)r[d] * (double)z[d];
-      rr += (double)r[d] * (double)r[d];
+      z[d + i] = zv[i];
+      rzn += (double)rv[i] * (double)zv[i];
+      rr += (double)rv[i] * (double)rv[i];
     }
   }
-  const double t0 = block_sum(rz, sh);
-  const double t1 = block_sum(rr, sh);
+  const double t0 = block_reduce_all<kT>(rzn, sh);
+  const double t1 = block_reduce_all<kT>(rr, sh);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = t0;
-    partials[2 * blockIdx.x + 1] = t1;
-  }
-  if (last_block(counter)) {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) {
-      a += partials[2 * i];
-      b += partials[2 * i + 1];
-    }
-    const double sa = block_sum(a, sh);
-    const double sb = block_sum(b, sh);
-    if (threadIdx.x == 0) {
-      scal[kBeta] = sa / scal[kRZ];
-      scal[kRZ] = sa;
-      scal[kRR] = sb;
-      *counter = 0;
-    }
+    P_rz_next[blockIdx.x] = t0;
+    P_rr[blockIdx.x] = t1;
   }
 }
 
-__global__ __launch_bounds__(kT) void k_pcg32_pupdate(float* p, const float* z, int ndof, const double* scal) {
-  const float beta = (float)scal[kBeta];
-  for (int d = blockIdx.x * kT + threadIdx.x; d < ndof; d += gridDim.x * kT) p[d] = z[d] + beta * p[d];
-}
-
-__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const double* b, const uint8_t* bcmask,
-                                                     float* r32, int nn, double* partials, unsigned* counter,
-                                                     double* scal, int slot) {
+__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* __restrict__ R, const double* __restrict__ b,
+                                                     const uint8_t* __restrict__ bcmask, float* __restrict__ r32,
+                                                     int nn, double* __restrict__ P) {
   __shared__ double sh[kT / 64];
   double rr = 0.0;
-  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+  const XcdSpan sp = xcd_span(nn);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
     const uint32_t bm = bcmask[n];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -329,17 +304,12 @@ __global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* R, const doub
       rr += v * v;
     }
   }
-  const double t = block_sum(rr, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-  if (last_block(counter)) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) v += partials[i];
-    const double tot = block_sum(v, sh);
-    if (threadIdx.x == 0) {
-      scal[slot] = tot;
-      *counter = 0;
-    }
-  }
+  const double t = block_reduce_all<kT>(rr, sh);
+  if (threadIdx.x == 0) P[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kT) void k_zero_f32(float* y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] = 0.0f;
 }
 
 __global__ __launch_bounds__(kT) void k_axpy_f32(double* u, const float* d, size_t n) {
@@ -359,9 +329,11 @@ int grid_of(size_t n, int cap = 1024) {
 
 }  // namespace
 
+int mf_apply_grid(int nchunks) { return red_grid((size_t)nchunks, 4); }
+
 hipError_t launch_mf_apply(const MfArgs& a, hipStream_t s) {
   const size_t lds = sizeof(float4) * a.max_nodes + sizeof(float) * 3 * (a.max_nodes + a.max_rows);
-  k_mf_apply<256><<<a.nchunks, 256, lds, s>>>(a);
+  k_mf_apply<256><<<mf_apply_grid(a.nchunks), 256, lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -370,23 +342,24 @@ hipError_t launch_mf_diag_inv(const DiagArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_pcg32_init(const float* r, float* z, float* p, const float* dinv, int nn, double* partials,
-                             unsigned* counter, double* scal, hipStream_t s) {
-  k_pcg32_init<<<grid_of(nn), kT, 0, s>>>(r, z, p, dinv, nn, partials, counter, scal);
+hipError_t launch_pcg32_init(const float* r, float* z, const float* dinv, int nn, double* P_rz, int grid,
+                             hipStream_t s) {
+  k_pcg32_init<<<grid, kT, 0, s>>>(r, z, dinv, nn, P_rz);
   return hipGetLastError();
 }
-hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, const float* q, const float* dinv, int nn,
-                               double* partials, unsigned* counter, double* scal, hipStream_t s) {
-  k_pcg32_update<<<grid_of(nn), kT, 0, s>>>(x, r, z, p, q, dinv, nn, partials, counter, scal);
-  return hipGetLastError();
-}
-hipError_t launch_pcg32_pupdate(float* p, const float* z, int ndof, const double* scal, hipStream_t s) {
-  k_pcg32_pupdate<<<grid_of(ndof), kT, 0, s>>>(p, z, ndof, scal);
+hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, const float* q, const float* dinv,
+                               int nn, const double* P_pq, int np_pq, const double* P_rz, int np,
+                               double* P_rz_next, double* P_rr, double* scal, int grid, hipStream_t s) {
+  k_pcg32_update<<<grid, kT, 0, s>>>(x, r, z, p, q, dinv, nn, P_pq, np_pq, P_rz, np, P_rz_next, P_rr, scal);
   return hipGetLastError();
 }
 hipError_t launch_neg_mask_to_f32(const double* R, const double* b, const uint8_t* bcmask, float* r32, int nn,
-                                  double* partials, unsigned* counter, double* scal, int slot, hipStream_t s) {
-  k_neg_mask_f32<<<grid_of(nn), kT, 0, s>>>(R, b, bcmask, r32, nn, partials, counter, scal, slot);
+                                  double* P, int grid, hipStream_t s) {
+  k_neg_mask_f32<<<grid, kT, 0, s>>>(R, b, bcmask, r32, nn, P);
+  return hipGetLastError();
+}
+hipError_t launch_zero_f32(float* y, size_t n, hipStream_t s) {
+  k_zero_f32<<<grid_of(n, 4096), kT, 0, s>>>(y, n);
   return hipGetLastError();
 }
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nn, hipStream_t s) {

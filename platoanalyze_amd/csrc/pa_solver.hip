@@ -1,91 +1,56 @@
-// Block-CRS (3x3) kernels: SpMV, Jacobi-PCG steps with fused grid reductions,
+// Block-CRS (3x3) kernels: SpMV, Jacobi-PCG steps with fence-free grid reductions,
 // Dirichlet elimination and load application.
+#include "pa_reduce.hpp"
 #include "pa_solver.hpp"
 
 namespace pa {
 
 constexpr int kT = 256;
 constexpr int kLanesPerRow = 16;  // 4 block rows per wavefront
-
-// ------------------------------------------------------------------ reductions
-// Block sum of one value per thread (deterministic order).
-__device__ __forceinline__ double block_sum(double v, double* sh) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int q = 0; q < kT / 64; ++q) t += sh[q];
-  return t;  // valid in thread 0
-}
-
-// Returns true in every thread of the block that arrives last.  Partials written
-// before the call are visible to that block.  The counter is reset by the last block.
-__device__ __forceinline__ bool last_block(unsigned* counter) {
-  __shared__ int am_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (am_last) __threadfence();
-  return am_last != 0;
-}
-
-__device__ __forceinline__ double sum_partials(const double* p, int n, double* sh) {
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += kT) v += __builtin_nontemporal_load(p + i);
-  return block_sum(v, sh);
-}
+constexpr int kRowsPerBlock = kT / kLanesPerRow;
 
 // ------------------------------------------------------------------ SpMV
-// 16 lanes per block row walk its 9*len values contiguously; lane partials for the
-// three output components are reduced with cross-lane shuffles.
+// 16 lanes per block row; lane l takes the row's blocks l, l+16, ... (9 contiguous
+// values, one column index, three gathered x values each), so every lane issues all
+// of its loads back to back.  Row sums are reduced across the 16 lanes.
+// X(c, j) returns the j-th component of the input at node c.
+template <typename XF>
 __device__ __forceinline__ void spmv_row(const double* __restrict__ vals, const int* __restrict__ rowmap,
-                                         const int* __restrict__ col, const double* __restrict__ x, int row,
-                                         int lane, double acc[3]) {
-  acc[0] = acc[1] = acc[2] = 0.0;
+                                         const int* __restrict__ col, int row, int lane, const XF& X,
+                                         double acc[3]) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
   const int s0 = rowmap[row], s1 = rowmap[row + 1];
-  const int n = 9 * (s1 - s0);
-  const double* v = vals + 9 * (size_t)s0;
-  int k = lane;
-  int s = k / 9, t = k - 9 * s;
-  for (; k < n; k += kLanesPerRow) {
-    const int c = col[s0 + s];
-    const int i = t / 3, j = t - 3 * (t / 3);
-    const double p = v[k] * x[3 * (size_t)c + j];
-    if (i == 0) acc[0] += p;
-    else if (i == 1) acc[1] += p;
-    else acc[2] += p;
-    t += kLanesPerRow - 9;  // advance 16 = 9 + 7
-    s += 1;
-    if (t >= 9) {
-      t -= 9;
-      s += 1;
-    }
+  for (int s = s0 + lane; s < s1; s += kLanesPerRow) {
+    const int c = col[s];
+    const double* v = vals + 9 * (size_t)s;
+    const double x0 = X(c, 0), x1 = X(c, 1), x2 = X(c, 2);
+    a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+    a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+    a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
   }
 #pragma unroll
   for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-    acc[0] += __shfl_xor(acc[0], off, kLanesPerRow);
-    acc[1] += __shfl_xor(acc[1], off, kLanesPerRow);
-    acc[2] += __shfl_xor(acc[2], off, kLanesPerRow);
+    a0 += __shfl_xor(a0, off, kLanesPerRow);
+    a1 += __shfl_xor(a1, off, kLanesPerRow);
+    a2 += __shfl_xor(a2, off, kLanesPerRow);
   }
+  acc[0] = a0;
+  acc[1] = a1;
+  acc[2] = a2;
 }
 
 // y = A x (mode 0) or y = b - A x (mode 1)
-__global__ __launch_bounds__(kT) void k_spmv(const double* vals, const int* rowmap, const int* col,
-                                             const double* x, const double* b, double* y, int nrows,
+__global__ __launch_bounds__(kT) void k_spmv(const double* __restrict__ vals, const int* __restrict__ rowmap,
+                                             const int* __restrict__ col, const double* __restrict__ x,
+                                             const double* __restrict__ b, double* __restrict__ y, int nrows,
                                              int mode) {
   const int lane = threadIdx.x % kLanesPerRow;
-  const int rows_per_block = kT / kLanesPerRow;
-  for (int row = blockIdx.x * rows_per_block + threadIdx.x / kLanesPerRow; row < nrows;
-       row += gridDim.x * rows_per_block) {
+  const XcdSpan sp = xcd_span(nrows);
+  auto X = [&](int c, int j) { return x[3 * (size_t)c + j]; };
+  for (int row = sp.lo + sp.j * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < sp.hi;
+       row += sp.bx * kRowsPerBlock) {
     double acc[3];
-    spmv_row(vals, rowmap, col, x, row, lane, acc);
+    spmv_row(vals, rowmap, col, row, lane, X, acc);
     if (lane < 3) {
       const double a = lane == 0 ? acc[0] : (lane == 1 ? acc[1] : acc[2]);
       const size_t d = 3 * (size_t)row + lane;
@@ -94,165 +59,147 @@ __global__ __launch_bounds__(kT) void k_spmv(const double* vals, const int* rowm
   }
 }
 
-// q = A p, then alpha = rz / (p.q) by the last block.
-__global__ __launch_bounds__(kT) void k_pcg_spmv(const double* vals, const int* rowmap, const int* col,
-                                                 const double* p, double* q, int nrows, double* partials,
-                                                 unsigned* counter, double* scal) {
+// PCG direction update fused into the product (p ping-pongs between two buffers):
+//   beta = rz_cur / rz_prev (partials of the previous update kernel),
+//   p_cur = z + beta p_prev (just z on the first iteration), q = A p_cur,
+//   per-group partial of p_cur . q into P_pq.
+// Every row of the graph is written by exactly one row group, so p_cur is complete
+// when the kernel ends; the product gathers p_cur at the columns from z and p_prev.
+__global__ __launch_bounds__(kT) void k_pcg_spmv(const double* __restrict__ vals, const int* __restrict__ rowmap,
+                                                 const int* __restrict__ col, const double* __restrict__ z,
+                                                 const double* __restrict__ p_prev, double* __restrict__ p_cur,
+                                                 double* __restrict__ q, int nrows, const double* P_rz_cur,
+                                                 const double* P_rz_prev, int np, int first,
+                                                 double* __restrict__ P_pq) {
   __shared__ double sh[kT / 64];
+  double beta = 0.0;
+  if (!first) {
+    const double a = sum_partials_all<kT>(P_rz_cur, np, sh);
+    const double b = sum_partials_all<kT>(P_rz_prev, np, sh);
+    beta = a / b;
+  }
   const int lane = threadIdx.x % kLanesPerRow;
-  const int rows_per_block = kT / kLanesPerRow;
+  const XcdSpan sp = xcd_span(nrows);
+  auto X = [&](int c, int j) {
+    const size_t d = 3 * (size_t)c + j;
+    return first ? z[d] : fma(beta, p_prev[d], z[d]);
+  };
   double dot = 0.0;
-  for (int row = blockIdx.x * rows_per_block + threadIdx.x / kLanesPerRow; row < nrows;
-       row += gridDim.x * rows_per_block) {
+  for (int row = sp.lo + sp.j * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < sp.hi;
+       row += sp.bx * kRowsPerBlock) {
     double acc[3];
-    spmv_row(vals, rowmap, col, p, row, lane, acc);
+    spmv_row(vals, rowmap, col, row, lane, X, acc);
     if (lane < 3) {
       const double a = lane == 0 ? acc[0] : (lane == 1 ? acc[1] : acc[2]);
+      const double pc = X(row, lane);
       const size_t d = 3 * (size_t)row + lane;
+      p_cur[d] = pc;
       q[d] = a;
-      dot += a * p[d];
+      dot += pc * a;
     }
   }
-  const double t = block_sum(dot, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-  if (last_block(counter)) {
-    const double pq = sum_partials(partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[kPQ] = pq;
-      scal[kAlpha] = scal[kRZ] / pq;
-      *counter = 0;
-    }
-  }
+  const double t = block_reduce_all<kT>(dot, sh);
+  if (threadIdx.x == 0) P_pq[blockIdx.x] = t;
 }
 
-// z = Dinv r on 3x3 blocks
-__device__ __forceinline__ void apply_dinv(const double* dinv, const double* r, double* z, int n) {
-  const double* D = dinv + 9 * (size_t)n;
-  const double r0 = r[3 * (size_t)n], r1 = r[3 * (size_t)n + 1], r2 = r[3 * (size_t)n + 2];
-  z[3 * (size_t)n + 0] = D[0] * r0 + D[1] * r1 + D[2] * r2;
-  z[3 * (size_t)n + 1] = D[3] * r0 + D[4] * r1 + D[5] * r2;
-  z[3 * (size_t)n + 2] = D[6] * r0 + D[7] * r1 + D[8] * r2;
+// z = Dinv r on one 3x3 block
+__device__ __forceinline__ void dinv3(const double* __restrict__ D, const double r[3], double z[3]) {
+  z[0] = D[0] * r[0] + D[1] * r[1] + D[2] * r[2];
+  z[1] = D[3] * r[0] + D[4] * r[1] + D[5] * r[2];
+  z[2] = D[6] * r[0] + D[7] * r[1] + D[8] * r[2];
 }
 
-// x += alpha p; r -= alpha q; z = Dinv r; rz_new = r.z, rr = r.r; beta by the last block.
-__global__ __launch_bounds__(kT) void k_pcg_update(double* x, double* r, double* z, const double* p,
-                                                   const double* q, const double* dinv, int nnodes,
-                                                   double* partials, unsigned* counter, double* scal) {
+// alpha = rz / pq from the partials; x += alpha p; r -= alpha q; z = Dinv r;
+// partials of r.z (next rz) and r.r.  Group 0 records pq, rz, alpha in scal.
+__global__ __launch_bounds__(kT) void k_pcg_update(double* __restrict__ x, double* __restrict__ r,
+                                                   double* __restrict__ z, const double* __restrict__ p,
+                                                   const double* __restrict__ q, const double* __restrict__ dinv,
+                                                   int nnodes, const double* P_pq, int np_pq, const double* P_rz,
+                                                   int np, double* __restrict__ P_rz_next,
+                                                   double* __restrict__ P_rr, double* __restrict__ scal) {
   __shared__ double sh[kT / 64];
-  const double alpha = scal[kAlpha];
-  double rz = 0.0, rr = 0.0;
-  for (int n = blockIdx.x * kT + threadIdx.x; n < nnodes; n += gridDim.x * kT) {
+  const double pq = sum_partials_all<kT>(P_pq, np_pq, sh);
+  const double rz = sum_partials_all<kT>(P_rz, np, sh);
+  const double alpha = rz / pq;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[kPQ] = pq;
+    scal[kRZ] = rz;
+    scal[kAlpha] = alpha;
+  }
+  double rzn = 0.0, rr = 0.0;
+  const XcdSpan sp = xcd_span(nnodes);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
+    const size_t d = 3 * (size_t)n;
+    double rv[3], zv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      x[d] += alpha * p[d];
-      r[d] -= alpha * q[d];
+      x[d + i] += alpha * p[d + i];
+      rv[i] = r[d + i] - alpha * q[d + i];
+      r[d + i] = rv[i];
     }
-    apply_dinv(dinv, r, z, n);
+    dinv3(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      rz += r[d] * z[d];
-      rr += r[d] * r[d];
+      z[d + i] = zv[i];
+      rzn += rv[i] * zv[i];
+      rr += rv[i] * rv[i];
     }
   }
-  const double t0 = block_sum(rz, sh);
-  const double t1 = block_sum(rr, sh);
+  const double t0 = block_reduce_all<kT>(rzn, sh);
+  const double t1 = block_reduce_all<kT>(rr, sh);
   if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = t0;
-    partials[2 * blockIdx.x + 1] = t1;
-  }
-  if (last_block(counter)) {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) {
-      a += partials[2 * i];
-      b += partials[2 * i + 1];
-    }
-    const double sa = block_sum(a, sh);
-    const double sb = block_sum(b, sh);
-    if (threadIdx.x == 0) {
-      scal[kBeta] = sa / scal[kRZ];
-      scal[kRZ] = sa;
-      scal[kRR] = sb;
-      *counter = 0;
-    }
+    P_rz_next[blockIdx.x] = t0;
+    P_rr[blockIdx.x] = t1;
   }
 }
 
-__global__ __launch_bounds__(kT) void k_pcg_pupdate(double* p, const double* z, int ndof, const double* scal) {
-  const double beta = scal[kBeta];
-  for (int d = blockIdx.x * kT + threadIdx.x; d < ndof; d += gridDim.x * kT) p[d] = z[d] + beta * p[d];
-}
-
-// z = Dinv r, p = z, rz = r.z, rr = r.r, bb = b.b (start of PCG)
-__global__ __launch_bounds__(kT) void k_pcg_init(const double* r, double* z, double* p, const double* b,
-                                                 const double* dinv, int nnodes, double* partials,
-                                                 unsigned* counter, double* scal) {
+// Start of PCG: z = Dinv r; partials of r.z, r.r and b.b.
+__global__ __launch_bounds__(kT) void k_pcg_init(const double* __restrict__ r, double* __restrict__ z,
+                                                 const double* __restrict__ b, const double* __restrict__ dinv,
+                                                 int nnodes, double* __restrict__ P_rz, double* __restrict__ P_rr,
+                                                 double* __restrict__ P_bb) {
   __shared__ double sh[kT / 64];
   double rz = 0.0, rr = 0.0, bb = 0.0;
-  for (int n = blockIdx.x * kT + threadIdx.x; n < nnodes; n += gridDim.x * kT) {
-    apply_dinv(dinv, r, z, n);
+  const XcdSpan sp = xcd_span(nnodes);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
+    const size_t d = 3 * (size_t)n;
+    double rv[3] = {r[d], r[d + 1], r[d + 2]}, zv[3];
+    dinv3(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const size_t d = 3 * (size_t)n + i;
-      p[d] = z[d];
-      rz += r[d] * z[d];
-      rr += r[d] * r[d];
-      bb += b[d] * b[d];
+      z[d + i] = zv[i];
+      rz += rv[i] * zv[i];
+      rr += rv[i] * rv[i];
+      bb += b[d + i] * b[d + i];
     }
   }
-  const double t0 = block_sum(rz, sh);
-  const double t1 = block_sum(rr, sh);
-  const double t2 = block_sum(bb, sh);
+  const double t0 = block_reduce_all<kT>(rz, sh);
+  const double t1 = block_reduce_all<kT>(rr, sh);
+  const double t2 = block_reduce_all<kT>(bb, sh);
   if (threadIdx.x == 0) {
-    partials[3 * blockIdx.x] = t0;
-    partials[3 * blockIdx.x + 1] = t1;
-    partials[3 * blockIdx.x + 2] = t2;
-  }
-  if (last_block(counter)) {
-    double a = 0.0, b2 = 0.0, c = 0.0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += kT) {
-      a += partials[3 * i];
-      b2 += partials[3 * i + 1];
-      c += partials[3 * i + 2];
-    }
-    const double sa = block_sum(a, sh);
-    const double sb = block_sum(b2, sh);
-    const double sc = block_sum(c, sh);
-    if (threadIdx.x == 0) {
-      scal[kRZ] = sa;
-      scal[kRR] = sb;
-      scal[kBB] = sc;
-      *counter = 0;
-    }
+    P_rz[blockIdx.x] = t0;
+    P_rr[blockIdx.x] = t1;
+    P_bb[blockIdx.x] = t2;
   }
 }
 
-// generic dot products: scal[slot] = a.b  (and a.a into slot+1 when want_aa)
-__global__ __launch_bounds__(kT) void k_dot(const double* a, const double* b, size_t n, double* partials,
-                                            unsigned* counter, double* scal, int slot) {
+// per-group partials of a.b
+__global__ __launch_bounds__(kT) void k_dot(const double* __restrict__ a, const double* __restrict__ b, size_t n,
+                                            double* __restrict__ P) {
   __shared__ double sh[kT / 64];
   double v = 0.0;
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) v += a[i] * b[i];
-  const double t = block_sum(v, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = t;
-  if (last_block(counter)) {
-    const double s = sum_partials(partials, gridDim.x, sh);
-    if (threadIdx.x == 0) {
-      scal[slot] = s;
-      *counter = 0;
-    }
-  }
+  const double t = block_reduce_all<kT>(v, sh);
+  if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
-// sum of per-chunk partials into scal[slot]
+// fixed-order sum of n partials into scal[slot] (one work-group)
 __global__ __launch_bounds__(kT) void k_sum(const double* partials, int n, double* scal, int slot) {
   __shared__ double sh[kT / 64];
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += kT) v += partials[i];
-  const double t = block_sum(v, sh);
+  const double t = sum_partials_all<kT>(partials, n, sh);
   if (threadIdx.x == 0) scal[slot] = t;
 }
+
 
 // 3x3 block-diagonal inverse (cofactors)
 __global__ __launch_bounds__(kT) void k_diag_inv(const double* vals, const int* diag_slot, double* dinv,
