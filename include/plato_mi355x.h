@@ -131,14 +131,15 @@ typedef struct {
   double tolerance;   /* relative residual ||b-Ax||/||b|| to stop at                 */
   int max_iters;      /* iteration cap (PA_ERR_NOT_CONVERGED when reached)           */
   int check_every;    /* host convergence check period (iterations)                  */
-  int precision;      /* 0: FP64 block-CRS PCG; 1: FP32 matrix-free PCG + FP64 refinement */
+  int precision;      /* 0: FP64 block-CRS PCG; 1: FP64 PCG with an FP32 matrix-free operator */
   int use_x0;         /* 1: start from the contents of x (warm start), 0: from zero  */
-  double inner_tolerance; /* FP32 inner solve relative tolerance (precision 1)       */
+  double inner_tolerance; /* precision 1: drop of the iterated residual between true FP64
+                             residual replacements (clamped to [1e-6, 0.5]; 0 -> 0.1)     */
 } pa_solver_opts;
 
 typedef struct {
   int iterations;       /* total inner iterations                                    */
-  int outer_iterations; /* refinement sweeps (precision 1)                           */
+  int outer_iterations; /* FP64 residual replacements (precision 1)                  */
   int converged;
   double rel_residual;  /* final true FP64 ||b-Ax||/||b||                            */
 } pa_solver_stats;
@@ -155,10 +156,11 @@ int pa_solve(pa_handle h, const double* values, double* x, const double* b,
 /* Problem::solution (src/elliptic/Problem.hpp:259-311): u = 0, rhs = -R(0,z) = f,
  * K = gradient_u, constraints, solve, u += du.  `values` is caller scratch of 9*nnz_b
  * doubles (may be NULL for precision 1, which never forms K).
- * precision 1: FP64 iterative refinement -- true residual r = f - K u from the FP64
- * element kernel -- around an FP32 matrix-free Jacobi-PCG that solves each correction
- * to inner_tolerance; use_x0 = 1 starts from the caller's u (warm start; its
- * constrained dofs are reset to their prescribed values). */
+ * precision 1: Jacobi-PCG with FP64 vectors and reductions whose operator K p is applied
+ * matrix-free in FP32; each time the iterated residual has dropped by inner_tolerance it
+ * is replaced by the true FP64 residual f - K u of the FP64 element kernel (the search
+ * direction is kept); convergence is judged on that true residual.  use_x0 = 1 starts from the caller's u
+ * (warm start; its constrained dofs are reset to their prescribed values). */
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
 

@@ -97,13 +97,12 @@ struct pa_problem {
   double f_free_norm2 = 0.0;
   // mixed-precision solver workspace
   DevArray<float4> m_xz;
-  DevArray<float> m_dinv, m_r, m_z, m_p, m_p2, m_q, m_d;
-  DevArray<double> m_R;
+  DevArray<double> m_dinv, m_r, m_z, m_p, m_p2, m_q, m_R;
   DevArray<double> partials;  // kNumPartialSlots x kRedBlocks
   DevArray<double> scal;
   DevArray<double> chunk_partials;
   // solver workspace (lazily allocated)
-  DevArray<double> w_r, w_z, w_p, w_p2, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
+  DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -498,7 +497,6 @@ static int ensure_solver_ws(pa_handle h) {
     PA_HIP(h->w_r.alloc(ndof));
     PA_HIP(h->w_z.alloc(ndof));
     PA_HIP(h->w_p.alloc(ndof));
-    PA_HIP(h->w_p2.alloc(ndof));
     PA_HIP(h->w_q.alloc(ndof));
     PA_HIP(h->w_dinv.alloc(9 * (size_t)h->nn));
   }
@@ -516,7 +514,7 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
              pa_solver_stats* st) {
   if (int rc = ready(h)) return rc;
   if (!vals || !x || !b) return fail(PA_ERR_ARG, "pa_solve: null pointer");
-  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
+  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
   if (int rc = ensure_solver_ws(h)) return rc;
   const int nn = h->nn;
@@ -533,7 +531,7 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
     PA_HIP(hipMemcpyAsync(h->w_r.p, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  pa::k_pcg_init<<<G, 256, 0, s>>>(h->w_r.p, h->w_z.p, b, h->w_dinv.p, nn, PS(pa::kPRZ0), PS(pa::kPRR),
+  pa::k_pcg_init<<<G, 256, 0, s>>>(h->w_r.p, h->w_z.p, h->w_p.p, b, h->w_dinv.p, nn, PS(pa::kPRZ0), PS(pa::kPRR),
                                     PS(pa::kPBB));
   pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
   pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPBB), G, h->scal.p, pa::kBB);
@@ -543,15 +541,14 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   double rel = bb > 0 ? std::sqrt(h->host_scal[pa::kRR] / bb) : 0.0;
   int it = 0;
   bool conv = bb == 0.0 || rel <= opts.tolerance;
-  double* pbuf[2] = {h->w_p.p, h->w_p2.p};
   while (!conv && it < opts.max_iters) {
+    // rz of this iteration in P_RZ[cur]; the update writes the next one into P_RZ[cur ^ 1]
     const int cur = it & 1;
-    pa::k_pcg_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, h->w_z.p, pbuf[cur ^ 1], pbuf[cur],
-                                      h->w_q.p, nn, PS(pa::kPRZ0 + cur), PS(pa::kPRZ0 + (cur ^ 1)), G, it == 0,
-                                      PS(pa::kPPQ));
-    pa::k_pcg_update<<<G, 256, 0, s>>>(x, h->w_r.p, h->w_z.p, pbuf[cur], h->w_q.p, h->w_dinv.p, nn, PS(pa::kPPQ),
+    pa::k_pcg_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, h->w_p.p, h->w_q.p, nn, PS(pa::kPPQ));
+    pa::k_pcg_update<<<G, 256, 0, s>>>(x, h->w_r.p, h->w_z.p, h->w_p.p, h->w_q.p, h->w_dinv.p, nn, PS(pa::kPPQ),
                                        Gs, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRR),
                                        h->scal.p);
+    pa::k_pcg_pupdate<<<G, 256, 0, s>>>(h->w_p.p, h->w_z.p, nn, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRZ0 + cur), G);
     ++it;
     if (it % opts.check_every == 0 || it == opts.max_iters) {
       pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
@@ -599,7 +596,6 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     PA_HIP(h->m_p.alloc(ndof));
     PA_HIP(h->m_p2.alloc(ndof));
     PA_HIP(h->m_q.alloc(ndof));
-    PA_HIP(h->m_d.alloc(ndof));
     PA_HIP(h->m_R.alloc(ndof));
   }
   PA_HIP(pa::launch_make_xz(h->X.p, z, h->m_xz.p, nn, s));
@@ -639,13 +635,12 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   ma.bcmask = h->bcmask.p;
   ma.z = h->m_z.p;
   ma.y = h->m_q.p;
-  ma.np = G;
-  ma.P_pq = PS(pa::kPPQ);
+  ma.scal = h->scal.p;
+  ma.partials = h->chunk_partials.p;
   ma.m = h->material();
-  const int Gm = pa::mf_apply_grid(h->chunks.count);
-  // ||r||^2 of the masked FP32 conversion of b - R into scal[slot]
+  // r = mask(b - R) into m_r and ||r||^2
   auto masked_residual = [&](const double* R, double* out) -> int {
-    PA_HIP(pa::launch_neg_mask_to_f32(R, b, h->bcmask.p, h->m_r.p, nn, PS(pa::kPDot), G, s));
+    PA_HIP(pa::launch_neg_mask(R, b, h->bcmask.p, h->m_r.p, nn, PS(pa::kPDot), G, s));
     pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot1);
     PA_HIP(hipGetLastError());
     if (int rc = read_scal(h)) return rc;
@@ -657,50 +652,63 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(h->m_R.p, ndof, 0.0);
     if (int rc = masked_residual(h->m_R.p, &bb)) return rc;
   }
+  // Jacobi-PCG in FP64 whose operator K p is applied in FP32 (matrix-free); the
+  // iterated residual is replaced by the true FP64 residual f - K u whenever it has
+  // dropped by `delta` since the last replacement (the search direction is kept), and
+  // convergence is judged on that true residual.
   int inner_total = 0, outer = 0;
   double rel = 0.0;
   bool conv = false;
-  const double itol = opts.inner_tolerance > 0 ? opts.inner_tolerance : 1e-4;
-  float* pbuf[2] = {h->m_p.p, h->m_p2.p};
-  while (true) {
+  const double delta = std::min(0.5, std::max(1e-6, opts.inner_tolerance > 0 ? opts.inner_tolerance : 0.1));
+  double* pbuf[2] = {h->m_p.p, h->m_p2.p};
+  // true residual of u into m_r and its norm
+  auto true_residual = [&](double* rr) -> int {
     if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, b == nullptr)) return rc;
-    double rr0 = 0.0;
-    if (int rc = masked_residual(h->m_R.p, &rr0)) return rc;
-    rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
+    return masked_residual(h->m_R.p, rr);
+  };
+  double rr_ref = 0.0;
+  if (int rc = true_residual(&rr_ref)) return rc;
+  rel = bb > 0 ? std::sqrt(rr_ref / bb) : 0.0;
+  if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve: non-finite residual");
+  conv = bb == 0.0 || rel <= opts.tolerance;
+  if (!conv) {
+    PA_HIP(pa::launch_pcgm_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
+    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, pa::kBetaInit, s));
+  }
+  int it = 0;
+  while (!conv) {
+    const int cur = it & 1;
+    ma.first = it == 0;
+    ma.p_prev = pbuf[cur ^ 1];
+    ma.p_cur = pbuf[cur];
+    PA_HIP(pa::launch_mf_apply(ma, s));
+    PA_HIP(pa::launch_pcgm_alpha(h->chunk_partials.p, h->chunks.count, h->scal.p, s));
+    PA_HIP(pa::launch_pcgm_update(u, h->m_r.p, h->m_z.p, pbuf[cur], h->m_q.p, h->m_dinv.p, nn, h->scal.p,
+                                  PS(pa::kPRZ0), PS(pa::kPRR), G, s));
+    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), PS(pa::kPRR), G, h->scal.p, pa::kBetaStep, s));
+    ++it;
+    ++inner_total;
+    const bool last = inner_total >= opts.max_iters;
+    if (it % opts.check_every != 0 && !last) continue;
+    if (int rc = read_scal(h)) return rc;
+    const double rr_it = h->host_scal[pa::kRR];
+    if (!std::isfinite(rr_it)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision PCG: non-finite residual");
+    // replace when the iterated residual has dropped enough, or could already meet the
+    // tolerance, or at the iteration cap
+    const bool could_meet = rr_it <= opts.tolerance * opts.tolerance * bb;
+    if (!(rr_it <= delta * delta * rr_ref || could_meet || last)) continue;
+    ++outer;
+    if (int rc = true_residual(&rr_ref)) return rc;
+    rel = std::sqrt(rr_ref / bb);
     if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve: non-finite residual");
-    if (rel <= opts.tolerance || bb == 0.0) {
+    if (rel <= opts.tolerance) {
       conv = true;
       break;
     }
-    if (inner_total >= opts.max_iters || outer >= 50) break;
-    ++outer;
-    // inner FP32 PCG on K d = r from d = 0
-    PA_HIP(pa::launch_zero_f32(h->m_d.p, ndof, s));
-    PA_HIP(pa::launch_pcg32_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
-    int it = 0;
-    while (inner_total < opts.max_iters) {
-      const int cur = it & 1;
-      ma.first = it == 0;
-      ma.p_prev = pbuf[cur ^ 1];
-      ma.p_cur = pbuf[cur];
-      ma.P_rz_cur = PS(pa::kPRZ0 + cur);
-      ma.P_rz_prev = PS(pa::kPRZ0 + (cur ^ 1));
-      PA_HIP(pa::launch_mf_apply(ma, s));
-      PA_HIP(pa::launch_pcg32_update(h->m_d.p, h->m_r.p, h->m_z.p, pbuf[cur], h->m_q.p, h->m_dinv.p, nn,
-                                     PS(pa::kPPQ), Gm, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)),
-                                     PS(pa::kPRR), h->scal.p, G, s));
-      ++it;
-      ++inner_total;
-      if (it % opts.check_every == 0) {
-        pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
-        PA_HIP(hipGetLastError());
-        if (int rc = read_scal(h)) return rc;
-        const double ir = std::sqrt(h->host_scal[pa::kRR] / rr0);
-        if (!std::isfinite(ir)) return fail(PA_ERR_NOT_CONVERGED, "inner FP32 PCG: non-finite residual");
-        if (ir <= itol) break;
-      }
-    }
-    PA_HIP(pa::launch_axpy_f32(u, h->m_d.p, ndof, s));
+    if (last) break;
+    // continue from the replaced residual: z = Dinv r, beta = (r.z) / rz of the last step
+    PA_HIP(pa::launch_pcgm_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
+    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, pa::kBetaReplace, s));
   }
   if (st) {
     st->iterations = inner_total;
@@ -717,7 +725,7 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
                 pa_solver_stats* st) {
   if (int rc = ready(h)) return rc;
   if (!z || !u) return fail(PA_ERR_ARG, "pa_solution: null pointer");
-  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
+  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
   if (opts.precision == 1) return solve_mixed(h, z, u, nullptr, opts, st);
   if (opts.precision != 0) return fail(PA_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
@@ -800,7 +808,7 @@ int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int 
                              pa_solver_stats* st) {
   if (int rc = ready(h)) return rc;
   if (!u || !z || !dfdz || !lambda) return fail(PA_ERR_ARG, "pa_stress_pnorm_gradient: null pointer");
-  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 1e-4};
+  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
   if (int rc = ensure_solver_ws(h)) return rc;
   const size_t ndof = 3 * (size_t)h->nn;

@@ -1,5 +1,5 @@
-// Matrix-free FP32 operator, FP64 block-diagonal preconditioner and FP32 PCG steps
-// for the mixed-precision solve (inner FP32 PCG, outer FP64 iterative refinement).
+// Mixed-precision solve: FP32 matrix-free operator inside an FP64 Jacobi-PCG (FP64
+// vectors and reductions, residual replaced by the true FP64 residual f - K u).
 #include "pa_mf.hpp"
 #include "pa_reduce.hpp"
 #include "pa_solver.hpp"
@@ -47,103 +47,107 @@ __device__ __forceinline__ void gradients_t(const T x[4][3], T g[4][3], T& vol) 
   vol = fabs(det) * (T(1) / T(6));
 }
 
-// q = K p (FP32) with the PCG direction update fused in.  Persistent work-groups walk
-// the chunks of their XCD's contiguous share.  Per chunk: the node table (FP32
-// coordinates + density) and p = z + beta p_prev at the table's nodes are staged in
-// LDS (constrained dofs read as 0), the chunk's own rows of p are written out, the
-// element loop accumulates K_e p_e into the owned rows with ds_add_f32, and the rows
-// are streamed out with the p.q partial.
+// q = K p with the PCG direction update fused in; one work-group per chunk.  The
+// vectors are FP64; the element operator is FP32: the node table (FP32 coordinates +
+// density) and p = z + beta p_prev at the table's nodes (rounded to FP32, constrained
+// dofs 0) are staged in LDS, the element loop computes K_e p_e in FP32 and adds it into
+// FP64 row accumulators with ds_add_f64, and the rows are written back with the
+// chunk's p.q partial.
+// The chunk's own rows of p_cur are written in FP64 (unrounded).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
-  extern __shared__ float smem[];
-  float4* nd4 = reinterpret_cast<float4*>(smem);
+  // LDS: own-row p and row accumulators (FP64, 3 x even(max_rows) each: multiples of
+  // 16 bytes), node table, p at the table's nodes (FP32).  FP64 LDS atomics run at
+  // several times the rate of FP32 ones on this part, and add no rounding of their own.
+  extern __shared__ double smem_d[];
+  const int mr2 = (A.max_rows + 1) & ~1;
+  double* pown = smem_d;
+  double* acc = pown + 3 * mr2;
+  float4* nd4 = reinterpret_cast<float4*>(acc + 3 * mr2);
   float* xl = reinterpret_cast<float*>(nd4 + A.max_nodes);
-  float* acc = xl + 3 * A.max_nodes;
   __shared__ double sh[NT / 64];
 
-  float beta = 0.0f;
-  if (!A.first) {
-    const double a = sum_partials_all<NT>(A.P_rz_cur, A.np, sh);
-    const double b = sum_partials_all<NT>(A.P_rz_prev, A.np, sh);
-    beta = (float)(a / b);
+  const double beta = A.first ? 0.0 : A.scal[kBeta];
+  const int c = xcd_map(blockIdx.x, A.nchunks);
+  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
+  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+  for (int k = threadIdx.x; k < nnode; k += NT) {
+    const int gid = A.nodes[n0 + k];
+    nd4[k] = A.Xz[gid];
+    const uint32_t bm = A.bcmask[gid];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)gid + i;
+      double v = A.first ? A.z[d] : fma(beta, A.p_prev[d], A.z[d]);
+      v = (bm >> i) & 1u ? 0.0 : v;
+      xl[3 * k + i] = (float)v;
+      if (k < nr) {  // rows come first in the table
+        A.p_cur[d] = v;
+        pown[3 * k + i] = v;
+      }
+    }
   }
+  for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0;
+  __syncthreads();
+
   const float c00 = (float)A.m.c00, c01 = (float)A.m.c01, c33 = (float)A.m.c33;
   const float rmin = (float)A.m.rmin, pp = (float)A.m.p;
+  const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+  for (int k = l0 + threadIdx.x; k < l1; k += NT) {
+    const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
+    const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
+    float x[4][3], g[4][3], u[4][3], vol, rho = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const float4 p = nd4[ln[a]];
+      x[a][0] = p.x;
+      x[a][1] = p.y;
+      x[a][2] = p.z;
+      rho += p.w;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u[a][i] = xl[3 * ln[a] + i];
+    }
+    gradients_t(x, g, vol);
+    rho *= 0.25f;
+    const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
+    float e[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      e[0] += u[a][0] * g[a][0];
+      e[1] += u[a][1] * g[a][1];
+      e[2] += u[a][2] * g[a][2];
+      e[3] += u[a][2] * g[a][1] + u[a][1] * g[a][2];
+      e[4] += u[a][2] * g[a][0] + u[a][0] * g[a][2];
+      e[5] += u[a][1] * g[a][0] + u[a][0] * g[a][1];
+    }
+    const float s0 = vw * (c00 * e[0] + c01 * e[1] + c01 * e[2]);
+    const float s1 = vw * (c01 * e[0] + c00 * e[1] + c01 * e[2]);
+    const float s2 = vw * (c01 * e[0] + c01 * e[1] + c00 * e[2]);
+    const float s3 = vw * c33 * e[3], s4 = vw * c33 * e[4], s5 = vw * c33 * e[5];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (ln[a] < nr) {
+        double* dst = &acc[3 * ln[a]];
+        __hip_atomic_fetch_add(dst + 0, (double)(s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(dst + 1, (double)(s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(dst + 2, (double)(s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    }
+  }
+  __syncthreads();
   double dot = 0.0;
-  const XcdSpan sp = xcd_span(A.nchunks);
-  for (int c = sp.lo + sp.j; c < sp.hi; c += sp.bx) {
-    const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
-    const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
-    for (int k = threadIdx.x; k < nnode; k += NT) {
-      const int gid = A.nodes[n0 + k];
-      nd4[k] = A.Xz[gid];
-      const uint32_t bm = A.bcmask[gid];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        const size_t d = 3 * (size_t)gid + i;
-        float v = A.first ? A.z[d] : fmaf(beta, A.p_prev[d], A.z[d]);
-        v = (bm >> i) & 1u ? 0.0f : v;
-        xl[3 * k + i] = v;
-        if (k < nr) A.p_cur[d] = v;  // rows come first in the table
-      }
-    }
-    for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0f;
-    __syncthreads();
-
-    const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
-    for (int k = l0 + threadIdx.x; k < l1; k += NT) {
-      const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
-      const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
-      float x[4][3], g[4][3], u[4][3], vol, rho = 0.0f;
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const float4 p = nd4[ln[a]];
-        x[a][0] = p.x;
-        x[a][1] = p.y;
-        x[a][2] = p.z;
-        rho += p.w;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) u[a][i] = xl[3 * ln[a] + i];
-      }
-      gradients_t(x, g, vol);
-      rho *= 0.25f;
-      const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
-      float e[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        e[0] += u[a][0] * g[a][0];
-        e[1] += u[a][1] * g[a][1];
-        e[2] += u[a][2] * g[a][2];
-        e[3] += u[a][2] * g[a][1] + u[a][1] * g[a][2];
-        e[4] += u[a][2] * g[a][0] + u[a][0] * g[a][2];
-        e[5] += u[a][1] * g[a][0] + u[a][0] * g[a][1];
-      }
-      const float s0 = vw * (c00 * e[0] + c01 * e[1] + c01 * e[2]);
-      const float s1 = vw * (c01 * e[0] + c00 * e[1] + c01 * e[2]);
-      const float s2 = vw * (c01 * e[0] + c01 * e[1] + c00 * e[2]);
-      const float s3 = vw * c33 * e[3], s4 = vw * c33 * e[4], s5 = vw * c33 * e[5];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        if (ln[a] < nr) {
-          float* dst = &acc[3 * ln[a]];
-          atomicAdd(dst + 0, s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]);
-          atomicAdd(dst + 1, s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]);
-          atomicAdd(dst + 2, s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]);
-        }
-      }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < 3 * nr; k += NT) {
-      const size_t gd = 3 * (size_t)r0 + k;
-      const float xv = xl[k];
-      const float yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? xv : acc[k];
-      A.y[gd] = yv;
-      dot += (double)xv * (double)yv;
-    }
-    __syncthreads();  // the next chunk reuses the LDS tables
+  for (int k = threadIdx.x; k < 3 * nr; k += NT) {
+    const size_t gd = 3 * (size_t)r0 + k;
+    const double pv = pown[k];
+    const double yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? pv : acc[k];
+    A.y[gd] = yv;
+    dot += pv * yv;
   }
   const double t = block_reduce_all<NT>(dot, sh);
-  if (threadIdx.x == 0) A.P_pq[blockIdx.x] = t;
+  if (threadIdx.x == 0) A.partials[c] = t;
 }
 
 // Diagonal blocks K_rr (FP64) accumulated per owned vertex, then inverted per row.
@@ -206,91 +210,112 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
         if (((bm >> i) | (bm >> j)) & 1u) a[3 * i + j] = (i == j) ? 1.0 : 0.0;
     const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
     const double id = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
-    float* o = A.dinv + 9 * (size_t)(r0 + lr);
-    o[0] = (float)(c00 * id);
-    o[1] = (float)((a[2] * a[7] - a[1] * a[8]) * id);
-    o[2] = (float)((a[1] * a[5] - a[2] * a[4]) * id);
-    o[3] = (float)(c01 * id);
-    o[4] = (float)((a[0] * a[8] - a[2] * a[6]) * id);
-    o[5] = (float)((a[2] * a[3] - a[0] * a[5]) * id);
-    o[6] = (float)(c02 * id);
-    o[7] = (float)((a[1] * a[6] - a[0] * a[7]) * id);
-    o[8] = (float)((a[0] * a[4] - a[1] * a[3]) * id);
+    double* o = A.dinv + 9 * (size_t)(r0 + lr);
+    o[0] = c00 * id;
+    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+    o[3] = c01 * id;
+    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+    o[6] = c02 * id;
+    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
   }
 }
 
-__device__ __forceinline__ void dinv3f(const float* __restrict__ D, const float r[3], float z[3]) {
+__device__ __forceinline__ void dinv3(const double* __restrict__ D, const double r[3], double z[3]) {
   z[0] = D[0] * r[0] + D[1] * r[1] + D[2] * r[2];
   z[1] = D[3] * r[0] + D[4] * r[1] + D[5] * r[2];
   z[2] = D[6] * r[0] + D[7] * r[1] + D[8] * r[2];
 }
 
-__global__ __launch_bounds__(kT) void k_pcg32_init(const float* __restrict__ r, float* __restrict__ z,
-                                                   const float* __restrict__ dinv, int nn,
-                                                   double* __restrict__ P_rz) {
+// z = Dinv r; partials of r.z
+__global__ __launch_bounds__(kT) void k_pcgm_init(const double* __restrict__ r, double* __restrict__ z,
+                                                  const double* __restrict__ dinv, int nn,
+                                                  double* __restrict__ P_rz) {
   __shared__ double sh[kT / 64];
   double rz = 0.0;
   const XcdSpan sp = xcd_span(nn);
   for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
     const size_t d = 3 * (size_t)n;
-    const float rv[3] = {r[d], r[d + 1], r[d + 2]};
-    float zv[3];
-    dinv3f(dinv + 9 * (size_t)n, rv, zv);
+    const double rv[3] = {r[d], r[d + 1], r[d + 2]};
+    double zv[3];
+    dinv3(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       z[d + i] = zv[i];
-      rz += (double)rv[i] * (double)zv[i];
+      rz += rv[i] * zv[i];
     }
   }
   const double t = block_reduce_all<kT>(rz, sh);
   if (threadIdx.x == 0) P_rz[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kT) void k_pcg32_update(float* __restrict__ x, float* __restrict__ r,
-                                                     float* __restrict__ z, const float* __restrict__ p,
-                                                     const float* __restrict__ q, const float* __restrict__ dinv,
-                                                     int nn, const double* P_pq, int np_pq, const double* P_rz,
-                                                     int np, double* __restrict__ P_rz_next,
-                                                     double* __restrict__ P_rr, double* __restrict__ scal) {
+__global__ __launch_bounds__(kT) void k_pcgm_alpha(const double* partials, int n, double* scal) {
   __shared__ double sh[kT / 64];
-  const double pq = sum_partials_all<kT>(P_pq, np_pq, sh);
-  const double rz = sum_partials_all<kT>(P_rz, np, sh);
-  const float alpha = (float)(rz / pq);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const double pq = sum_partials_all<kT>(partials, n, sh);
+  if (threadIdx.x == 0) {
     scal[kPQ] = pq;
-    scal[kRZ] = rz;
-    scal[kAlpha] = alpha;
+    scal[kAlpha] = scal[kRZ] / pq;
   }
+}
+
+// x += alpha p; r -= alpha q; z = Dinv r (alpha = scal[kAlpha]); partials of r.z, r.r
+__global__ __launch_bounds__(kT) void k_pcgm_update(double* __restrict__ x, double* __restrict__ r,
+                                                    double* __restrict__ z, const double* __restrict__ p,
+                                                    const double* __restrict__ q, const double* __restrict__ dinv,
+                                                    int nn, const double* __restrict__ scal,
+                                                    double* __restrict__ P_rz, double* __restrict__ P_rr) {
+  __shared__ double sh[kT / 64];
+  const double alpha = scal[kAlpha];
   double rzn = 0.0, rr = 0.0;
   const XcdSpan sp = xcd_span(nn);
   for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
     const size_t d = 3 * (size_t)n;
-    float rv[3], zv[3];
+    double rv[3], zv[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       x[d + i] += alpha * p[d + i];
       rv[i] = r[d + i] - alpha * q[d + i];
       r[d + i] = rv[i];
     }
-    dinv3f(dinv + 9 * (size_t)n, rv, zv);
+    dinv3(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       z[d + i] = zv[i];
-      rzn += (double)rv[i] * (double)zv[i];
-      rr += (double)rv[i] * (double)rv[i];
+      rzn += rv[i] * zv[i];
+      rr += rv[i] * rv[i];
     }
   }
   const double t0 = block_reduce_all<kT>(rzn, sh);
   const double t1 = block_reduce_all<kT>(rr, sh);
   if (threadIdx.x == 0) {
-    P_rz_next[blockIdx.x] = t0;
+    P_rz[blockIdx.x] = t0;
     P_rr[blockIdx.x] = t1;
   }
 }
 
-__global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* __restrict__ R, const double* __restrict__ b,
-                                                     const uint8_t* __restrict__ bcmask, float* __restrict__ r32,
-                                                     int nn, double* __restrict__ P) {
+__global__ __launch_bounds__(kT) void k_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal,
+                                                  int mode) {
+  __shared__ double sh[kT / 64];
+  const double rz = sum_partials_all<kT>(P_rz, n, sh);
+  const double rr = P_rr ? sum_partials_all<kT>(P_rr, n, sh) : 0.0;
+  if (threadIdx.x == 0) {
+    if (mode == kBetaStep) {
+      scal[kBeta] = rz / scal[kRZ];
+      scal[kRZPrev] = scal[kRZ];
+    } else if (mode == kBetaReplace) {
+      scal[kBeta] = rz / scal[kRZPrev];
+    }
+    scal[kRZ] = rz;
+    if (P_rr) scal[kRR] = rr;
+  }
+}
+
+// r = mask(b - R) (b optional); partials of r.r
+__global__ __launch_bounds__(kT) void k_neg_mask(const double* __restrict__ R, const double* __restrict__ b,
+                                                 const uint8_t* __restrict__ bcmask, double* __restrict__ r, int nn,
+                                                 double* __restrict__ P) {
   __shared__ double sh[kT / 64];
   double rr = 0.0;
   const XcdSpan sp = xcd_span(nn);
@@ -300,20 +325,12 @@ __global__ __launch_bounds__(kT) void k_neg_mask_f32(const double* __restrict__ 
     for (int i = 0; i < 3; ++i) {
       const size_t d = 3 * (size_t)n + i;
       const double v = (bm >> i) & 1u ? 0.0 : (b ? b[d] : 0.0) - R[d];
-      r32[d] = (float)v;
+      r[d] = v;
       rr += v * v;
     }
   }
   const double t = block_reduce_all<kT>(rr, sh);
   if (threadIdx.x == 0) P[blockIdx.x] = t;
-}
-
-__global__ __launch_bounds__(kT) void k_zero_f32(float* y, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] = 0.0f;
-}
-
-__global__ __launch_bounds__(kT) void k_axpy_f32(double* u, const float* d, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) u[i] += (double)d[i];
 }
 
 __global__ __launch_bounds__(kT) void k_make_xz(const double* X, const double* z, float4* Xz, int nn) {
@@ -329,45 +346,41 @@ int grid_of(size_t n, int cap = 1024) {
 
 }  // namespace
 
-int mf_apply_grid(int nchunks) { return red_grid((size_t)nchunks, 4); }
-
 hipError_t launch_mf_apply(const MfArgs& a, hipStream_t s) {
-  const size_t lds = sizeof(float4) * a.max_nodes + sizeof(float) * 3 * (a.max_nodes + a.max_rows);
-  k_mf_apply<256><<<mf_apply_grid(a.nchunks), 256, lds, s>>>(a);
+  const size_t lds = sizeof(double) * 6 * ((a.max_rows + 1) & ~1) + sizeof(float4) * a.max_nodes +
+                     sizeof(float) * 3 * a.max_nodes;
+  k_mf_apply<256><<<a.nchunks, 256, lds, s>>>(a);
   return hipGetLastError();
 }
-
 hipError_t launch_mf_diag_inv(const DiagArgs& a, hipStream_t s) {
   k_mf_diag<256><<<a.nchunks, 256, sizeof(double) * 9 * a.max_rows, s>>>(a);
   return hipGetLastError();
 }
-
-hipError_t launch_pcg32_init(const float* r, float* z, const float* dinv, int nn, double* P_rz, int grid,
-                             hipStream_t s) {
-  k_pcg32_init<<<grid, kT, 0, s>>>(r, z, dinv, nn, P_rz);
+hipError_t launch_pcgm_init(const double* r, double* z, const double* dinv, int nn, double* P_rz, int grid,
+                            hipStream_t s) {
+  k_pcgm_init<<<grid, kT, 0, s>>>(r, z, dinv, nn, P_rz);
   return hipGetLastError();
 }
-hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, const float* q, const float* dinv,
-                               int nn, const double* P_pq, int np_pq, const double* P_rz, int np,
-                               double* P_rz_next, double* P_rr, double* scal, int grid, hipStream_t s) {
-  k_pcg32_update<<<grid, kT, 0, s>>>(x, r, z, p, q, dinv, nn, P_pq, np_pq, P_rz, np, P_rz_next, P_rr, scal);
+hipError_t launch_pcgm_alpha(const double* partials, int n, double* scal, hipStream_t s) {
+  k_pcgm_alpha<<<1, kT, 0, s>>>(partials, n, scal);
   return hipGetLastError();
 }
-hipError_t launch_neg_mask_to_f32(const double* R, const double* b, const uint8_t* bcmask, float* r32, int nn,
-                                  double* P, int grid, hipStream_t s) {
-  k_neg_mask_f32<<<grid, kT, 0, s>>>(R, b, bcmask, r32, nn, P);
+hipError_t launch_pcgm_update(double* x, double* r, double* z, const double* p, const double* q, const double* dinv,
+                              int nn, const double* scal, double* P_rz, double* P_rr, int grid, hipStream_t s) {
+  k_pcgm_update<<<grid, kT, 0, s>>>(x, r, z, p, q, dinv, nn, scal, P_rz, P_rr);
   return hipGetLastError();
 }
-hipError_t launch_zero_f32(float* y, size_t n, hipStream_t s) {
-  k_zero_f32<<<grid_of(n, 4096), kT, 0, s>>>(y, n);
+hipError_t launch_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal, int mode, hipStream_t s) {
+  k_pcgm_beta<<<1, kT, 0, s>>>(P_rz, P_rr, n, scal, mode);
+  return hipGetLastError();
+}
+hipError_t launch_neg_mask(const double* R, const double* b, const uint8_t* bcmask, double* r, int nn, double* P,
+                           int grid, hipStream_t s) {
+  k_neg_mask<<<grid, kT, 0, s>>>(R, b, bcmask, r, nn, P);
   return hipGetLastError();
 }
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nn, hipStream_t s) {
   k_make_xz<<<grid_of(nn, 4096), kT, 0, s>>>(X, z, Xz, nn);
-  return hipGetLastError();
-}
-hipError_t launch_axpy_f32(double* u, const float* d, size_t n, hipStream_t s) {
-  k_axpy_f32<<<grid_of(n, 4096), kT, 0, s>>>(u, d, n);
   return hipGetLastError();
 }
 

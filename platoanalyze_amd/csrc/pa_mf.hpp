@@ -1,5 +1,6 @@
-// Matrix-free operators of the Tet4 elastostatics path (FP32 apply for the inner PCG
-// of the mixed-precision solve, FP64 block diagonal for its Jacobi preconditioner).
+// Mixed-precision operators of the Tet4 elastostatics path: an FP32 matrix-free K p
+// inside an FP64 Jacobi-PCG (FP64 vectors, reductions and block-diagonal
+// preconditioner; residual replaced by the true FP64 residual f - K u).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,28 +18,22 @@ struct MfArgs {
   const uint16_t* ent_ln;
   const int32_t* list_offs;
   int nchunks, max_rows, max_nodes;
-  // data
   const float4* Xz;       // FP32 x, y, z, density per node
   const uint8_t* bcmask;  // bit i set: dof 3n+i constrained
-  // direction update fused into the apply: p_cur = z + beta p_prev (p_cur = z when
-  // first), beta = sum(P_rz_cur) / sum(P_rz_prev) over np partials
-  const float* z;
-  const float* p_prev;
-  float* p_cur;
-  const double* P_rz_cur;
-  const double* P_rz_prev;
-  int np, first;
-  float* y;     // y = K p_cur (constrained rows: y = p_cur)
-  double* P_pq; // per-work-group partial of p_cur . y (FP64), gridDim.x of them
+  // direction update fused into the apply: p_cur = z + beta p_prev with beta =
+  // scal[kBeta] (p_cur = z when first); FP64 vectors
+  const double* z;
+  const double* p_prev;
+  double* p_cur;
+  int first;
+  const double* scal;
+  double* y;         // y = K p_cur through the FP32 operator (constrained rows: y = p_cur)
+  double* partials;  // per-chunk partial of p_cur . y
   Material m;
 };
-
-// grid of the persistent matrix-free apply (multiple of 8, <= kRedBlocks)
-int mf_apply_grid(int nchunks);
 hipError_t launch_mf_apply(const MfArgs& a, hipStream_t s);
 
-// FP64 3x3 diagonal blocks of K(z) per node, inverted (constrained dofs -> identity),
-// written as FP32 (9 per node).
+// FP64 3x3 diagonal blocks of K(z) per node, inverted (constrained dofs -> identity).
 struct DiagArgs {
   const int32_t* chunk_rows;
   const int32_t* node_offs;
@@ -49,31 +44,33 @@ struct DiagArgs {
   const double* X;
   const double* z;
   const uint8_t* bcmask;
-  float* dinv;
+  double* dinv;
   Material m;
 };
 hipError_t launch_mf_diag_inv(const DiagArgs& a, hipStream_t s);
 
-// FP32 PCG vector steps with FP64 partials (the FP64 solver's protocol: each kernel
-// writes per-work-group partials that the next kernel sums in a fixed order).
+// PCG steps.  Vector kernels write per-work-group partials; one-work-group finishers
+// turn them into the scalars of the next step.
 // z = Dinv r, partials of r.z into P_rz.
-hipError_t launch_pcg32_init(const float* r, float* z, const float* dinv, int nnodes, double* P_rz, int grid,
-                             hipStream_t s);
-// alpha = sum(P_pq)/sum(P_rz); x += alpha p; r -= alpha q; z = Dinv r; partials of
-// r.z and r.r.
-hipError_t launch_pcg32_update(float* x, float* r, float* z, const float* p, const float* q, const float* dinv,
-                               int nnodes, const double* P_pq, int np_pq, const double* P_rz, int np,
-                               double* P_rz_next, double* P_rr, double* scal, int grid, hipStream_t s);
-
-// r32 = mask(b - R) with b optional (R = K u - f from the residual kernel, or K x);
-// partials of ||r||^2 (FP64) into P, one per work-group of `grid`
-hipError_t launch_neg_mask_to_f32(const double* R, const double* b, const uint8_t* bcmask, float* r32, int nnodes,
-                                  double* P, int grid, hipStream_t s);
+hipError_t launch_pcgm_init(const double* r, double* z, const double* dinv, int nnodes, double* P_rz, int grid,
+                            hipStream_t s);
+// pq = sum of the n apply partials; scal[kAlpha] = scal[kRZ] / pq
+hipError_t launch_pcgm_alpha(const double* partials, int n, double* scal, hipStream_t s);
+// x += alpha p; r -= alpha q; z = Dinv r (alpha = scal[kAlpha]); partials of r.z, r.r
+hipError_t launch_pcgm_update(double* x, double* r, double* z, const double* p, const double* q, const double* dinv,
+                              int nnodes, const double* scal, double* P_rz, double* P_rr, int grid, hipStream_t s);
+// rz_new = sum P_rz; scal[kRR] = sum P_rr (when P_rr is given), then by mode:
+//   kBetaInit:    scal[kRZ] = rz_new
+//   kBetaStep:    scal[kBeta] = rz_new / scal[kRZ]; scal[kRZPrev] = scal[kRZ]; scal[kRZ] = rz_new
+//   kBetaReplace: scal[kBeta] = rz_new / scal[kRZPrev]; scal[kRZ] = rz_new  (rz_new of a
+//                 replaced residual)
+enum BetaMode { kBetaStep = 0, kBetaInit = 1, kBetaReplace = 2 };
+hipError_t launch_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal, int mode, hipStream_t s);
+// r = mask(b - R) with b optional (R = K u - f from the residual kernel, or K u);
+// partials of ||r||^2 into P, one per work-group of `grid`
+hipError_t launch_neg_mask(const double* R, const double* b, const uint8_t* bcmask, double* r, int nnodes, double* P,
+                           int grid, hipStream_t s);
 // Xz[n] = (float) (X[3n], X[3n+1], X[3n+2], z[n])
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nnodes, hipStream_t s);
-// y = 0 (FP32)
-hipError_t launch_zero_f32(float* y, size_t n, hipStream_t s);
-// u += d (FP64 += FP32)
-hipError_t launch_axpy_f32(double* u, const float* d, size_t n, hipStream_t s);
 
 }  // namespace pa

@@ -59,31 +59,14 @@ __global__ __launch_bounds__(kT) void k_spmv(const double* __restrict__ vals, co
   }
 }
 
-// PCG direction update fused into the product (p ping-pongs between two buffers):
-//   beta = rz_cur / rz_prev (partials of the previous update kernel),
-//   p_cur = z + beta p_prev (just z on the first iteration), q = A p_cur,
-//   per-group partial of p_cur . q into P_pq.
-// Every row of the graph is written by exactly one row group, so p_cur is complete
-// when the kernel ends; the product gathers p_cur at the columns from z and p_prev.
+// q = A p; per-group partial of p.q into P_pq.
 __global__ __launch_bounds__(kT) void k_pcg_spmv(const double* __restrict__ vals, const int* __restrict__ rowmap,
-                                                 const int* __restrict__ col, const double* __restrict__ z,
-                                                 const double* __restrict__ p_prev, double* __restrict__ p_cur,
-                                                 double* __restrict__ q, int nrows, const double* P_rz_cur,
-                                                 const double* P_rz_prev, int np, int first,
-                                                 double* __restrict__ P_pq) {
+                                                 const int* __restrict__ col, const double* __restrict__ p,
+                                                 double* __restrict__ q, int nrows, double* __restrict__ P_pq) {
   __shared__ double sh[kT / 64];
-  double beta = 0.0;
-  if (!first) {
-    const double a = sum_partials_all<kT>(P_rz_cur, np, sh);
-    const double b = sum_partials_all<kT>(P_rz_prev, np, sh);
-    beta = a / b;
-  }
   const int lane = threadIdx.x % kLanesPerRow;
   const XcdSpan sp = xcd_span(nrows);
-  auto X = [&](int c, int j) {
-    const size_t d = 3 * (size_t)c + j;
-    return first ? z[d] : fma(beta, p_prev[d], z[d]);
-  };
+  auto X = [&](int c, int j) { return p[3 * (size_t)c + j]; };
   double dot = 0.0;
   for (int row = sp.lo + sp.j * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < sp.hi;
        row += sp.bx * kRowsPerBlock) {
@@ -91,15 +74,29 @@ __global__ __launch_bounds__(kT) void k_pcg_spmv(const double* __restrict__ vals
     spmv_row(vals, rowmap, col, row, lane, X, acc);
     if (lane < 3) {
       const double a = lane == 0 ? acc[0] : (lane == 1 ? acc[1] : acc[2]);
-      const double pc = X(row, lane);
       const size_t d = 3 * (size_t)row + lane;
-      p_cur[d] = pc;
       q[d] = a;
-      dot += pc * a;
+      dot += p[d] * a;
     }
   }
   const double t = block_reduce_all<kT>(dot, sh);
   if (threadIdx.x == 0) P_pq[blockIdx.x] = t;
+}
+
+// beta = rz_new / rz_old from the partials of the last two update kernels; p = z + beta p.
+__global__ __launch_bounds__(kT) void k_pcg_pupdate(double* __restrict__ p, const double* __restrict__ z,
+                                                    int nnodes, const double* P_rz_new, const double* P_rz_old,
+                                                    int np) {
+  __shared__ double sh[kT / 64];
+  const double a = sum_partials_all<kT>(P_rz_new, np, sh);
+  const double b = sum_partials_all<kT>(P_rz_old, np, sh);
+  const double beta = a / b;
+  const XcdSpan sp = xcd_span(nnodes);
+  for (int n = sp.lo + sp.j * kT + threadIdx.x; n < sp.hi; n += sp.bx * kT) {
+    const size_t d = 3 * (size_t)n;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) p[d + i] = fma(beta, p[d + i], z[d + i]);
+  }
 }
 
 // z = Dinv r on one 3x3 block
@@ -153,10 +150,11 @@ __global__ __launch_bounds__(kT) void k_pcg_update(double* __restrict__ x, doubl
   }
 }
 
-// Start of PCG: z = Dinv r; partials of r.z, r.r and b.b.
+// Start of PCG: z = Dinv r, p = z; partials of r.z, r.r and b.b.
 __global__ __launch_bounds__(kT) void k_pcg_init(const double* __restrict__ r, double* __restrict__ z,
-                                                 const double* __restrict__ b, const double* __restrict__ dinv,
-                                                 int nnodes, double* __restrict__ P_rz, double* __restrict__ P_rr,
+                                                 double* __restrict__ p, const double* __restrict__ b,
+                                                 const double* __restrict__ dinv, int nnodes,
+                                                 double* __restrict__ P_rz, double* __restrict__ P_rr,
                                                  double* __restrict__ P_bb) {
   __shared__ double sh[kT / 64];
   double rz = 0.0, rr = 0.0, bb = 0.0;
@@ -168,6 +166,7 @@ __global__ __launch_bounds__(kT) void k_pcg_init(const double* __restrict__ r, d
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       z[d + i] = zv[i];
+      p[d + i] = zv[i];
       rz += rv[i] * zv[i];
       rr += rv[i] * rv[i];
       bb += b[d + i] * b[d + i];

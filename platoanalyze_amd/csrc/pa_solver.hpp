@@ -6,21 +6,22 @@ namespace pa {
 
 // slots of the device scalar array used by the PCG kernels
 enum ScalarSlot { kRZ = 0, kPQ = 1, kAlpha = 2, kBeta = 3, kRR = 4, kBB = 5, kDot0 = 6, kDot1 = 7,
-                  kEnergy = 8, kNumScal = 16 };
+                  kEnergy = 8, kRZPrev = 9, kNumScal = 16 };
 
 // partial arrays of the solver (kRedBlocks doubles each) in one device buffer
 enum PartialSlot { kPPQ = 0, kPRZ0 = 1, kPRZ1 = 2, kPRR = 3, kPBB = 4, kPDot = 5, kNumPartialSlots = 6 };
 
 __global__ void k_spmv(const double* vals, const int* rowmap, const int* col, const double* x,
                        const double* b, double* y, int nrows, int mode);
-__global__ void k_pcg_spmv(const double* vals, const int* rowmap, const int* col, const double* z,
-                           const double* p_prev, double* p_cur, double* q, int nrows, const double* P_rz_cur,
-                           const double* P_rz_prev, int np, int first, double* P_pq);
+__global__ void k_pcg_spmv(const double* vals, const int* rowmap, const int* col, const double* p, double* q,
+                           int nrows, double* P_pq);
+__global__ void k_pcg_pupdate(double* p, const double* z, int nnodes, const double* P_rz_new,
+                              const double* P_rz_old, int np);
 __global__ void k_pcg_update(double* x, double* r, double* z, const double* p, const double* q,
                              const double* dinv, int nnodes, const double* P_pq, int np_pq, const double* P_rz,
                              int np, double* P_rz_next, double* P_rr, double* scal);
-__global__ void k_pcg_init(const double* r, double* z, const double* b, const double* dinv, int nnodes,
-                           double* P_rz, double* P_rr, double* P_bb);
+__global__ void k_pcg_init(const double* r, double* z, double* p, const double* b, const double* dinv,
+                           int nnodes, double* P_rz, double* P_rr, double* P_bb);
 __global__ void k_dot(const double* a, const double* b, size_t n, double* P);
 __global__ void k_sum(const double* partials, int n, double* scal, int slot);
 __global__ void k_diag_inv(const double* vals, const int* diag_slot, double* dinv, int nnodes);

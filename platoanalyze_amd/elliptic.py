@@ -204,10 +204,10 @@ class Problem:
     """Plato::Elliptic::Problem (self-adjoint compliance)."""
 
     def __init__(self, model: ElasticModel, tolerance: float = 1e-10, max_iters: int = 200000,
-                 check_every: int = 16, precision: str = "fp64", inner_tolerance: float = 1e-4):
-        """precision "fp64": assembled block-CRS Jacobi-PCG in FP64.  "mixed": FP64
-        iterative refinement (true residual f - K u) around an FP32 matrix-free
-        Jacobi-PCG solving each correction to `inner_tolerance`."""
+                 check_every: int = 16, precision: str = "fp64", inner_tolerance: float = 0.1):
+        """precision "fp64": assembled block-CRS Jacobi-PCG in FP64.  "mixed": Jacobi-PCG
+        with FP64 vectors and an FP32 matrix-free operator, its residual replaced by the
+        true FP64 residual f - K u each time it has dropped by `inner_tolerance`."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.m = model

@@ -286,8 +286,8 @@ def test_gpu_scale_cfg1_parity():
 
 # ------------------------------------------------------------------ mixed precision
 def test_gpu_mixed_precision_solve_cfg0():
-    """FP32 matrix-free PCG inside FP64 iterative refinement reaches the FP64 tolerance;
-    errors against the FP64 block-CRS PCG solution of the same system are bounded."""
+    """FP32 matrix-free PCG with FP64 residual replacement reaches the FP64 tolerance on
+    the true residual; errors against the FP64 block-CRS PCG solution are bounded."""
     print("case: cfg0 mixed precision", flush=True)
     w = wl.make("cantilever_128x64x64", tiny=(20, 10, 10))
     m = wl.model(w)
@@ -296,10 +296,10 @@ def test_gpu_mixed_precision_solve_cfg0():
     u64 = host(ref.solution(dz))
     g64, C64 = ref.criterionGradient(dz, dev(u64))
     g64 = host(g64)
-    mp = Problem(m, tolerance=1e-10, precision="mixed", inner_tolerance=1e-4)
+    mp = Problem(m, tolerance=1e-10, precision="mixed", inner_tolerance=0.1)
     u = mp.solution(dz)
     assert mp.stats.converged and mp.stats.rel_residual <= 1e-10
-    assert mp.stats.outer_iterations >= 2  # refinement actually iterated
+    assert mp.stats.outer_iterations >= 2  # the residual was actually replaced
     uh = host(u)
     err_u = np.abs(uh - u64).max() / np.abs(u64).max()
     g, C = mp.criterionGradient(dz, u)
@@ -367,14 +367,16 @@ def test_gpu_dRdz_T_apply_cfg0(cfg0):
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_gpu_pnorm_total_gradient_fd_and_autograd(precision):
     """Hand-written adjoint of the relaxed von Mises p-norm (p=8, q=0.5) through a
-    torch.autograd.Function against central finite differences (h=1e-6) of full solves."""
+    torch.autograd.Function, with forward and adjoint solves in the precision under test,
+    against central finite differences (h=1e-6) of full FP64 solves: the FD reference
+    needs F to ~1e-13, which only the FP64 solve delivers without FP32 rounding noise."""
     from platoanalyze_amd import criteria
 
     w = wl.make("cantilever_20x10x10", tiny=(8, 4, 4))
     m = wl.model(w)
     z0 = np.random.default_rng(13).uniform(0.3, 1.0, w.mesh.num_nodes)
-    tol = 1e-12 if precision == "fp64" else 1e-11
-    pb = Problem(m, tolerance=tol, precision=precision, inner_tolerance=1e-5)
+    pb = Problem(m, tolerance=1e-12, precision=precision, inner_tolerance=0.1)
+    ref = Problem(m, tolerance=1e-13)
     z = dev(z0).requires_grad_(True)
     F = criteria.stress_pnorm(z, pb, 8.0, "von_mises", 0.5)
     F.backward()
@@ -386,8 +388,9 @@ def test_gpu_pnorm_total_gradient_fd_and_autograd(precision):
         zp, zm = z0.copy(), z0.copy()
         zp[n] += h
         zm[n] -= h
-        fp = float(criteria.stress_pnorm(dev(zp), pb, 8.0, "von_mises", 0.5))
-        fm = float(criteria.stress_pnorm(dev(zm), pb, 8.0, "von_mises", 0.5))
+        fp = float(criteria.stress_pnorm(dev(zp), ref, 8.0, "von_mises", 0.5))
+        fm = float(criteria.stress_pnorm(dev(zm), ref, 8.0, "von_mises", 0.5))
         fd = (fp - fm) / (2 * h)
+        print(f"node {n}: FD {fd:.10e} adjoint {g[n]:.10e}", flush=True)
         assert abs(fd - g[n]) <= 1e-6 * np.abs(g).max() + 1e-7 * abs(fd), (n, fd, g[n])
     assert float(C) > 0
