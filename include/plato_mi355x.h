@@ -20,6 +20,7 @@
 #ifndef PLATO_MI355X_H
 #define PLATO_MI355X_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -193,6 +194,30 @@ int pa_dRdz_T_apply(pa_handle h, const double* u, const double* lambda, const do
 int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int measure, double p, double q,
                              double* dfdz, double* lambda, double* value, const pa_solver_opts* opts,
                              pa_solver_stats* stats);
+
+/* ------------------------------------------------------------ learned warm start
+ * No counterpart in the reference (whose Problem::solution always starts from u = 0):
+ * a 4-level 3-D convolutional U-Net on the node grid of a box mesh, trained by the
+ * library's own BF16-MFMA forward/backward kernels, whose output warm-starts the solve
+ * (pa_solution with opts.use_x0 = 1).  Tensors are device FP32, channels last,
+ * [batch][nx][ny][nz][8]: input channels density, load (3), coordinates (3), 1;
+ * output channels u (3) then 5 padding channels (trained towards 0).  Node grid
+ * dimensions must be 1 mod 4.  Parameters: FP32 vector of pa_unet_num_params values
+ * (per layer: W[27][cin][cout] then bias[cout]). */
+typedef struct pa_unet_s* pa_unet;
+const char* pa_unet_last_error(void);
+int pa_unet_create(pa_unet* out, int nx, int ny, int nz, int base_channels, unsigned long long seed);
+int pa_unet_destroy(pa_unet h);
+int pa_unet_set_stream(pa_unet h, void* hip_stream);
+int pa_unet_num_params(pa_unet h, size_t* n);
+/* copy parameters out to dst and/or in from src (device pointers, either may be NULL) */
+int pa_unet_params(pa_unet h, float* dst, const float* src);
+int pa_unet_forward(pa_unet h, const float* x, int batch, float* y);
+/* mean-squared-error loss over all output channels and its gradient w.r.t. the
+ * parameters (grad may be NULL); loss on the host */
+int pa_unet_gradient(pa_unet h, const float* x, const float* target, int batch, float* grad, double* loss);
+/* pa_unet_gradient followed by one Adam step (beta 0.9, 0.999, eps 1e-8) at rate lr */
+int pa_unet_train_step(pa_unet h, const float* x, const float* target, int batch, float lr, double* loss);
 
 #ifdef __cplusplus
 }

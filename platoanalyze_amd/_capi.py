@@ -72,6 +72,15 @@ _SIGS = {
     "pa_dRdz_T_apply": (_I, [_P, _P, _P, _P, _P]),
     "pa_stress_pnorm_gradient": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _PD, C.POINTER(SolverOpts),
                                       C.POINTER(SolverStats)]),
+    "pa_unet_last_error": (C.c_char_p, []),
+    "pa_unet_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, C.c_ulonglong]),
+    "pa_unet_destroy": (_I, [_P]),
+    "pa_unet_set_stream": (_I, [_P, _P]),
+    "pa_unet_num_params": (_I, [_P, C.POINTER(C.c_size_t)]),
+    "pa_unet_params": (_I, [_P, _P, _P]),
+    "pa_unet_forward": (_I, [_P, _P, _I, _P]),
+    "pa_unet_gradient": (_I, [_P, _P, _P, _I, _P, _PD]),
+    "pa_unet_train_step": (_I, [_P, _P, _P, _I, C.c_float, _PD]),
 }
 
 _lib = None
@@ -106,3 +115,10 @@ def check(status: int, where: str) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args), name)
+
+
+def ucall(name: str, *args) -> None:
+    """Call a pa_unet_* entry point (errors from pa_unet_last_error)."""
+    status = getattr(lib(), name)(*args)
+    if status != PA_OK:
+        raise PaError(status, name, lib().pa_unet_last_error().decode())

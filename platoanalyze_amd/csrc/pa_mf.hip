@@ -251,9 +251,11 @@ __global__ __launch_bounds__(kT) void k_pcgm_init(const double* __restrict__ r, 
   if (threadIdx.x == 0) P_rz[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kT) void k_pcgm_alpha(const double* partials, int n, double* scal) {
-  __shared__ double sh[kT / 64];
-  const double pq = sum_partials_all<kT>(partials, n, sh);
+constexpr int kFin = 1024;  // threads of the one-work-group finishers
+
+__global__ __launch_bounds__(kFin) void k_pcgm_alpha(const double* partials, int n, double* scal) {
+  __shared__ double sh[kFin / 64];
+  const double pq = sum_partials_wide<kFin>(partials, n, sh);
   if (threadIdx.x == 0) {
     scal[kPQ] = pq;
     scal[kAlpha] = scal[kRZ] / pq;
@@ -295,11 +297,11 @@ __global__ __launch_bounds__(kT) void k_pcgm_update(double* __restrict__ x, doub
   }
 }
 
-__global__ __launch_bounds__(kT) void k_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal,
-                                                  int mode) {
-  __shared__ double sh[kT / 64];
-  const double rz = sum_partials_all<kT>(P_rz, n, sh);
-  const double rr = P_rr ? sum_partials_all<kT>(P_rr, n, sh) : 0.0;
+__global__ __launch_bounds__(kFin) void k_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal,
+                                                    int mode) {
+  __shared__ double sh[kFin / 64];
+  const double rz = sum_partials_wide<kFin>(P_rz, n, sh);
+  const double rr = P_rr ? sum_partials_wide<kFin>(P_rr, n, sh) : 0.0;
   if (threadIdx.x == 0) {
     if (mode == kBetaStep) {
       scal[kBeta] = rz / scal[kRZ];
@@ -362,7 +364,7 @@ hipError_t launch_pcgm_init(const double* r, double* z, const double* dinv, int 
   return hipGetLastError();
 }
 hipError_t launch_pcgm_alpha(const double* partials, int n, double* scal, hipStream_t s) {
-  k_pcgm_alpha<<<1, kT, 0, s>>>(partials, n, scal);
+  k_pcgm_alpha<<<1, kFin, 0, s>>>(partials, n, scal);
   return hipGetLastError();
 }
 hipError_t launch_pcgm_update(double* x, double* r, double* z, const double* p, const double* q, const double* dinv,
@@ -371,7 +373,7 @@ hipError_t launch_pcgm_update(double* x, double* r, double* z, const double* p, 
   return hipGetLastError();
 }
 hipError_t launch_pcgm_beta(const double* P_rz, const double* P_rr, int n, double* scal, int mode, hipStream_t s) {
-  k_pcgm_beta<<<1, kT, 0, s>>>(P_rz, P_rr, n, scal, mode);
+  k_pcgm_beta<<<1, kFin, 0, s>>>(P_rz, P_rr, n, scal, mode);
   return hipGetLastError();
 }
 hipError_t launch_neg_mask(const double* R, const double* b, const uint8_t* bcmask, double* r, int nn, double* P,

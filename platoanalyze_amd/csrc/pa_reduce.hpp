@@ -48,6 +48,22 @@ __device__ __forceinline__ double sum_partials_all(const double* p, int n, doubl
   return block_reduce_all<NT>(v, sh);
 }
 
+// Same, with four independent accumulators per thread (for long partial arrays read by
+// a single work-group: more loads in flight).  Fixed order, deterministic.
+template <int NT>
+__device__ __forceinline__ double sum_partials_wide(const double* p, int n, double* sh) {
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 3 * NT < n; i += 4 * NT) {
+    a0 += p[i];
+    a1 += p[i + NT];
+    a2 += p[i + 2 * NT];
+    a3 += p[i + 3 * NT];
+  }
+  for (; i < n; i += NT) a0 += p[i];
+  return block_reduce_all<NT>((a0 + a1) + (a2 + a3), sh);
+}
+
 // Contiguous share [lo, hi) of n items for the XCD that runs this work-group
 // (work-group b runs on XCD b % 8), and this group's rank j among the bx groups
 // of that XCD.  Items close in index (Hilbert-ordered rows, chunks) stay in one L2.
