@@ -1,0 +1,240 @@
+"""Learned warm start for Problem::solution on box meshes.
+
+The U-Net (platoanalyze_amd.unet, library kernels) maps (density, load, coordinates)
+on the node grid to the displacement made dimensionless by the load: u is linear in f,
+so the target is u * c0 / max|f_node| (c0 a constant fixed from the training set) and
+the prediction is scaled back with the load of the problem at hand.  The solve then
+starts from u0 (opts.use_x0); plain PCG from zero remains the fallback, and the true
+FP64 residual judges the result either way.  (Rescaling the predicted shape by the
+Galerkin factor f.u / u.K u instead is energy-optimal but is dominated by the
+prediction's high-frequency noise, which Jacobi-PCG removes cheaply anyway; it is kept
+as `galerkin_scale` for diagnostics.)
+
+Training data come from SIMP optimality-criteria trajectories solved by the library's
+own FP64 path: random volume fraction in [0.3, 0.5] and random unit traction direction
+on the x = L face per trajectory.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import time
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _capi
+from . import workloads as wl
+from .elliptic import Problem, _dptr
+from .unet import IN_CH, OUT_CH, UNet
+
+
+# ------------------------------------------------------------------ node grid
+def grid_index(w: wl.Workload) -> tuple[tuple[int, int, int], np.ndarray]:
+    """Node-grid dims and, per mesh node, its voxel index (ix * ny + iy) * nz + iz."""
+    c = w.cfg
+    dims = (c.nx + 1, c.ny + 1, c.nz + 1)
+    X = w.mesh.coords
+    ijk = np.rint(X / np.array([c.lx / c.nx, c.ly / c.ny, c.lz / c.nz])).astype(np.int64)
+    if np.any(ijk < 0) or np.any(ijk >= np.array(dims)):
+        raise ValueError("mesh is not a box node grid")
+    vox = (ijk[:, 0] * dims[1] + ijk[:, 1]) * dims[2] + ijk[:, 2]
+    if len(np.unique(vox)) != len(vox):
+        raise ValueError("mesh is not a box node grid")
+    return dims, vox
+
+
+def load_vector(model, z: torch.Tensor) -> torch.Tensor:
+    """f = -R(0, z): the assembled traction load (3N, device)."""
+    from .elliptic import VectorFunction
+
+    return -VectorFunction(model).value(torch.zeros(model.num_dofs, dtype=torch.float64, device=z.device), z)
+
+
+class Features:
+    """Input tensor builder for one workload (fixed mesh)."""
+
+    def __init__(self, w: wl.Workload, device="cuda"):
+        self.p, self.rmin = w.cfg.p, w.cfg.rho_min
+        self.dims, vox = grid_index(w)
+        self.vox = torch.as_tensor(vox, device=device)
+        self.V = int(np.prod(self.dims))
+        X = torch.as_tensor(w.mesh.coords, dtype=torch.float32, device=device)
+        self.xyz = X / torch.as_tensor([w.cfg.lx, w.cfg.lx, w.cfg.lx], dtype=torch.float32, device=device)
+
+    def inputs(self, z: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
+        x = torch.zeros((self.V, IN_CH), dtype=torch.float32, device=z.device)
+        fn = f.reshape(-1, 3).to(torch.float32)
+        fn = fn / fn.abs().max().clamp_min(1e-30)
+        x[self.vox, 0] = z.to(torch.float32)
+        x[self.vox, 1:4] = fn
+        x[self.vox, 4:7] = self.xyz
+        x[self.vox, 7] = (self.rmin + (1.0 - self.rmin) * z.pow(self.p)).to(torch.float32)  # SIMP stiffness
+        return x.reshape(*self.dims, IN_CH)
+
+    @staticmethod
+    def load_scale(f: torch.Tensor) -> float:
+        return float(f.reshape(-1, 3).abs().max().clamp_min(1e-300))
+
+    def target(self, u: torch.Tensor, f: torch.Tensor, c0: float) -> torch.Tensor:
+        un = (u * (c0 / self.load_scale(f))).reshape(-1, 3).to(torch.float32)
+        y = torch.zeros((self.V, OUT_CH), dtype=torch.float32, device=u.device)
+        y[self.vox, 0:3] = un
+        return y.reshape(*self.dims, OUT_CH)
+
+    def displacement(self, y: torch.Tensor, f: torch.Tensor, c0: float) -> torch.Tensor:
+        """Network output -> displacement in node order (3N, float64)."""
+        u = y.reshape(self.V, OUT_CH)[self.vox, 0:3].reshape(-1).to(torch.float64)
+        return u * (self.load_scale(f) / c0)
+
+
+def galerkin_scale(model, z: torch.Tensor, u_hat: torch.Tensor, f: torch.Tensor, bc_dofs: torch.Tensor) -> float:
+    """alpha minimising the energy 1/2 a^2 u.K u - a f.u over multiples of u_hat."""
+    from .elliptic import VectorFunction
+
+    u = u_hat.clone()
+    u[bc_dofs] = 0.0
+    Ku = VectorFunction(model).value(u, z) + f  # R(u) = K u - f
+    num = float(torch.dot(f, u))
+    den = float(torch.dot(u, Ku))
+    return num / den if den > 0 else 0.0
+
+
+class WarmStart:
+    """Trained network + the per-mesh feature map; `initial_guess` gives u0.
+    mode "load": the network predicts the load-scaled displacement (c0 > 0);
+    mode "shape": it predicts the unit-RMS shape and the amplitude is the Galerkin
+    factor (c0 <= 0)."""
+
+    def __init__(self, net: UNet, w: wl.Workload, model, c0: float):
+        self.net = net
+        self.c0 = float(c0)
+        self.mode = "load" if self.c0 > 0 else "shape"
+        self.w = w
+        self.model = model
+        self.feat = Features(w)
+        if tuple(self.feat.dims) != tuple(net.grid):
+            raise ValueError(f"network grid {net.grid} != mesh node grid {self.feat.dims}")
+        self.bc = torch.as_tensor(w.bc_dofs, dtype=torch.long, device="cuda")
+
+    def initial_guess(self, z: torch.Tensor, f: torch.Tensor | None = None) -> torch.Tensor:
+        if f is None:
+            f = load_vector(self.model, z)
+        x = self.feat.inputs(z, f).unsqueeze(0).contiguous()
+        y = self.net.forward(x)
+        if self.mode == "load":
+            u0 = self.feat.displacement(y, f, self.c0)
+            u0[self.bc] = 0.0
+            return u0
+        u_hat = self.feat.displacement(y, torch.ones(3, dtype=torch.float64, device=z.device), 1.0)
+        u_hat[self.bc] = 0.0
+        return galerkin_scale(self.model, z, u_hat, f, self.bc) * u_hat
+
+
+def save(net: UNet, c0: float, path: str | Path, meta: dict) -> None:
+    p = net.params().cpu().numpy()
+    np.savez(path, params=p, grid=np.array(net.grid), base=np.array(net.base), c0=np.array(c0),
+             **{k: np.array(v) for k, v in meta.items()})
+
+
+def load(path: str | Path, grid: tuple | None = None) -> tuple[UNet, float]:
+    """Load (network, c0); `grid` re-instantiates the (fully convolutional) network on
+    another node grid."""
+    d = np.load(path, allow_pickle=False)
+    g = tuple(int(v) for v in (grid if grid is not None else d["grid"]))
+    net = UNet(g, int(d["base"]))
+    net.set_params(torch.as_tensor(d["params"].astype(np.float32)))
+    return net, float(d["c0"])
+
+
+# ------------------------------------------------------------------ training data
+def nodal_volumes(w: wl.Workload) -> np.ndarray:
+    X, T = w.mesh.coords, w.mesh.conn
+    a, b, c, d = (X[T[:, k]] for k in range(4))
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)) / 6.0
+    wn = np.zeros(len(X))
+    for k in range(4):
+        np.add.at(wn, T[:, k], vol / 4.0)
+    return wn
+
+
+def oc_update(z: np.ndarray, dc: np.ndarray, wn: np.ndarray, vf: float, rho_min: float, move: float = 0.2):
+    """Optimality-criteria step on nodal densities with a volume constraint."""
+    lo, hi = 1e-9, 1e9
+    dcn = np.maximum(-dc, 1e-30) / wn
+    target = vf * wn.sum()
+    for _ in range(80):
+        lam = np.sqrt(lo * hi)
+        zn = np.clip(z * np.sqrt(dcn / lam), np.maximum(rho_min, z - move), np.minimum(1.0, z + move))
+        if (zn * wn).sum() > target:
+            lo = lam
+        else:
+            hi = lam
+        if hi / lo < 1 + 1e-9:
+            break
+    return zn
+
+
+def trajectories(base: wl.Workload, n_traj: int, n_iter: int, seed: int = 7, tol: float = 1e-8, log=print):
+    """Yield (trajectory, iteration, workload, model, z, f, u) along SIMP OC runs."""
+    rng = np.random.default_rng(seed)
+    wn = nodal_volumes(base)
+    for t in range(n_traj):
+        vf = float(rng.uniform(0.3, 0.5))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        w = dataclasses.replace(base, traction=tuple(float(v) for v in d))
+        m = wl.model(w)
+        pb = Problem(m, tolerance=tol, check_every=32)
+        z = np.full(base.mesh.num_nodes, vf)
+        zt = torch.as_tensor(z, device="cuda")
+        f = load_vector(m, zt)
+        t0 = time.perf_counter()
+        for k in range(n_iter):
+            zt = torch.as_tensor(z, device="cuda")
+            u = pb.solution(zt)
+            g, Cv = pb.criterionGradient(zt, u)
+            yield t, k, w, m, zt, f, u
+            z = oc_update(z, g.cpu().numpy(), wn, vf, base.cfg.rho_min)
+        log(f"trajectory {t}: vf {vf:.3f} dir {np.round(d, 3)} compliance {Cv:.4g} "
+            f"({time.perf_counter() - t0:.1f} s)")
+
+
+def train(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch: int = 4, lr: float = 1e-3,
+          base_channels: int = 16, seed: int = 0, holdout: int = 1, mode: str = "load", log=print):
+    """Generate data along trajectories, train, and return (net, c0, history, held-out
+    set); c0 <= 0 marks the "shape" mode (see WarmStart)."""
+    feat = Features(base)
+    X, U, held = [], [], []
+    for t, k, w, m, z, f, u in trajectories(base, n_traj + holdout, n_iter, seed=seed + 11, log=log):
+        if t < n_traj:
+            X.append(feat.inputs(z, f))
+            U.append((u / feat.load_scale(f)).to(torch.float32))
+        else:
+            held.append((w, z.clone(), f.clone()))
+    if mode == "shape":  # unit RMS per sample
+        U = [v / v.pow(2).mean().sqrt().clamp_min(1e-30) for v in U]
+        c0 = 1.0
+    else:  # c0: unit RMS of the load-scaled displacement over the training set
+        c0 = float(1.0 / torch.stack([v.pow(2).mean() for v in U]).mean().sqrt())
+    X = torch.stack(X)
+    Y = torch.stack([feat.target(v.to(torch.float64), torch.ones(3, device=v.device, dtype=torch.float64), c0)
+                     for v in U])
+    del U
+    net = UNet(feat.dims, base_channels, seed=seed)
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = X.shape[0]
+    for ep in range(epochs):
+        lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs, 1)))  # cosine decay
+        perm = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        t0 = time.perf_counter()
+        for i in range(0, n - batch + 1, batch):
+            idx = torch.as_tensor(perm[i:i + batch], device="cuda")
+            tot += net.train_step(X[idx].contiguous(), Y[idx].contiguous(), lr_ep)
+            cnt += 1
+        hist.append(tot / max(cnt, 1))
+        log(f"epoch {ep}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s, {cnt} steps)")
+    return net, (c0 if mode == "load" else -1.0), hist, held
