@@ -68,6 +68,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for scaling HBM counters)")
+    ap.add_argument("--profile-solve", type=int, default=0, metavar="K",
+                    help="run only the FP64 and mixed solves, each capped at K iterations (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the assembly/solve path runs on a single MI355X "
@@ -90,6 +92,24 @@ def main() -> int:
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": args.warmup + args.steps}))
+        return 0
+    if args.profile_solve:
+        w = wl.make(args.config)
+        m = wl.model(w, device=dev)
+        z = torch.as_tensor(w.z, device=dev)
+        out = {}
+        for prec in ("fp64", "mixed"):
+            pb = Problem(m, tolerance=1e-8, check_every=32, precision=prec, max_iters=args.profile_solve)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            try:
+                pb.solution(z)
+            except _capi.PaError as e:  # the cap is expected to stop the solve
+                if e.status != 5:
+                    raise
+            torch.cuda.synchronize()
+            out[prec] = {"seconds": time.perf_counter() - t0, "iterations": pb.stats.iterations}
+        print(json.dumps({"profile_solve": out, "config": args.config}))
         return 0
     t0 = time.perf_counter()
     w = wl.make(args.config)

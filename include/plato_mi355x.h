@@ -165,6 +165,13 @@ int pa_solve(pa_handle h, const double* values, double* x, const double* b,
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
 
+/* Damped block-Jacobi smoothing of the homogeneous problem, u <- u - omega D^-1 K(z) u
+ * at the free dofs, `steps` times (D: 3x3 diagonal blocks of K).  A low-pass filter
+ * adapted to K: modes with a large Rayleigh quotient are damped, smooth ones kept.  No
+ * counterpart in the reference; used to denoise a learned initial guess before its
+ * Galerkin scaling. */
+int pa_smooth(pa_handle h, const double* z, double* u, int steps, double omega);
+
 /* Problem::criterionGradient for the self-adjoint compliance
  * (src/elliptic/Problem.hpp:413-468, :437-441): dC/dz = -partial dC/dz at the solved u.
  * `value` (host, may be NULL) receives C. */

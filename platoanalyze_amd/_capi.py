@@ -72,6 +72,7 @@ _SIGS = {
     "pa_dRdz_T_apply": (_I, [_P, _P, _P, _P, _P]),
     "pa_stress_pnorm_gradient": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _PD, C.POINTER(SolverOpts),
                                       C.POINTER(SolverStats)]),
+    "pa_smooth": (_I, [_P, _P, _P, _I, _D]),
     "pa_unet_last_error": (C.c_char_p, []),
     "pa_unet_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, C.c_ulonglong]),
     "pa_unet_destroy": (_I, [_P]),

@@ -583,8 +583,8 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
 // inner: K d = r to inner_tolerance with y = K x in FP32; u += d in FP64.
 // b == nullptr: right-hand side is the traction load f; else b (device, 3N; constrained
 // entries ignored).
-static int solve_mixed(pa_handle h, const double* z, double* u, const double* b, const pa_solver_opts& opts,
-                       pa_solver_stats* st) {
+// workspace of the matrix-free paths and the FP64 block-diagonal inverse of K(z)
+static int mixed_setup(pa_handle h, const double* z) {
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
   hipStream_t s = h->stream;
@@ -614,6 +614,15 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   da.dinv = h->m_dinv.p;
   da.m = h->material();
   PA_HIP(pa::launch_mf_diag_inv(da, s));
+  return PA_OK;
+}
+
+static int solve_mixed(pa_handle h, const double* z, double* u, const double* b, const pa_solver_opts& opts,
+                       pa_solver_stats* st) {
+  const int nn = h->nn;
+  const size_t ndof = 3 * (size_t)nn;
+  hipStream_t s = h->stream;
+  if (int rc = mixed_setup(h, z)) return rc;
   if (!opts.use_x0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(u, ndof, 0.0);
   const int nbc = (int)h->bc_dofs.size();
   // prescribed values for the state; homogeneous constraints for a caller right-hand side
@@ -718,6 +727,18 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   }
   if (!conv)
     return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve stopped at relative residual " + std::to_string(rel));
+  return PA_OK;
+}
+
+int pa_smooth(pa_handle h, const double* z, double* u, int steps, double omega) {
+  if (int rc = ready(h)) return rc;
+  if (!z || !u) return fail(PA_ERR_ARG, "pa_smooth: null pointer");
+  if (steps < 0 || !(omega > 0.0 && omega < 2.0)) return fail(PA_ERR_ARG, "pa_smooth: steps >= 0, 0 < omega < 2");
+  if (int rc = mixed_setup(h, z)) return rc;
+  for (int k = 0; k < steps; ++k) {
+    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, false)) return rc;  // K u
+    PA_HIP(pa::launch_jacobi_step(u, h->m_R.p, h->m_dinv.p, h->bcmask.p, h->nn, omega, h->stream));
+  }
   return PA_OK;
 }
 

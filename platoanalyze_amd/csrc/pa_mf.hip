@@ -335,6 +335,21 @@ __global__ __launch_bounds__(kT) void k_neg_mask(const double* __restrict__ R, c
   if (threadIdx.x == 0) P[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(kT) void k_jacobi_step(double* __restrict__ u, const double* __restrict__ Ku,
+                                                    const double* __restrict__ dinv, const uint8_t* __restrict__ bcmask,
+                                                    int nn, double omega) {
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
+    const size_t d = 3 * (size_t)n;
+    const double r[3] = {Ku[d], Ku[d + 1], Ku[d + 2]};
+    double c[3];
+    dinv3(dinv + 9 * (size_t)n, r, c);
+    const uint32_t bm = bcmask[n];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (!((bm >> i) & 1u)) u[d + i] -= omega * c[i];
+  }
+}
+
 __global__ __launch_bounds__(kT) void k_make_xz(const double* X, const double* z, float4* Xz, int nn) {
   for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT)
     Xz[n] = make_float4((float)X[3 * (size_t)n], (float)X[3 * (size_t)n + 1], (float)X[3 * (size_t)n + 2],
@@ -379,6 +394,11 @@ hipError_t launch_pcgm_beta(const double* P_rz, const double* P_rr, int n, doubl
 hipError_t launch_neg_mask(const double* R, const double* b, const uint8_t* bcmask, double* r, int nn, double* P,
                            int grid, hipStream_t s) {
   k_neg_mask<<<grid, kT, 0, s>>>(R, b, bcmask, r, nn, P);
+  return hipGetLastError();
+}
+hipError_t launch_jacobi_step(double* u, const double* Ku, const double* dinv, const uint8_t* bcmask, int nn,
+                              double omega, hipStream_t s) {
+  k_jacobi_step<<<grid_of(nn, 4096), kT, 0, s>>>(u, Ku, dinv, bcmask, nn, omega);
   return hipGetLastError();
 }
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nn, hipStream_t s) {

@@ -70,6 +70,9 @@ hipError_t launch_pcgm_beta(const double* P_rz, const double* P_rr, int n, doubl
 // partials of ||r||^2 into P, one per work-group of `grid`
 hipError_t launch_neg_mask(const double* R, const double* b, const uint8_t* bcmask, double* r, int nnodes, double* P,
                            int grid, hipStream_t s);
+// u -= omega Dinv (K u) at the free dofs (Ku from the residual kernel without load)
+hipError_t launch_jacobi_step(double* u, const double* Ku, const double* dinv, const uint8_t* bcmask, int nnodes,
+                              double omega, hipStream_t s);
 // Xz[n] = (float) (X[3n], X[3n+1], X[3n+2], z[n])
 hipError_t launch_make_xz(const double* X, const double* z, float4* Xz, int nnodes, hipStream_t s);
 

@@ -101,14 +101,21 @@ def galerkin_scale(model, z: torch.Tensor, u_hat: torch.Tensor, f: torch.Tensor,
     return num / den if den > 0 else 0.0
 
 
+def smooth(model, z: torch.Tensor, u: torch.Tensor, steps: int, omega: float = 0.6) -> None:
+    """In place: `steps` damped block-Jacobi sweeps of the homogeneous problem (pa_smooth)."""
+    model._stream()
+    _capi.call("pa_smooth", model.handle, _dptr(z), _dptr(u), int(steps), float(omega))
+
+
 class WarmStart:
     """Trained network + the per-mesh feature map; `initial_guess` gives u0.
     mode "load": the network predicts the load-scaled displacement (c0 > 0);
     mode "shape": it predicts the unit-RMS shape and the amplitude is the Galerkin
     factor (c0 <= 0)."""
 
-    def __init__(self, net: UNet, w: wl.Workload, model, c0: float):
+    def __init__(self, net: UNet, w: wl.Workload, model, c0: float, smooth_steps: int = 8):
         self.net = net
+        self.smooth_steps = int(smooth_steps)
         self.c0 = float(c0)
         self.mode = "load" if self.c0 > 0 else "shape"
         self.w = w
@@ -129,6 +136,8 @@ class WarmStart:
             return u0
         u_hat = self.feat.displacement(y, torch.ones(3, dtype=torch.float64, device=z.device), 1.0)
         u_hat[self.bc] = 0.0
+        if self.smooth_steps:
+            smooth(self.model, z, u_hat, self.smooth_steps)
         return galerkin_scale(self.model, z, u_hat, f, self.bc) * u_hat
 
 

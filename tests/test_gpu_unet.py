@@ -72,3 +72,31 @@ def test_gpu_unet_training_reduces_loss():
         l1 = net.train_step(xt, yt, 1e-3)
     print(f"loss {l0:.4e} -> {l1:.4e}", flush=True)
     assert l1 < 0.25 * l0
+
+
+def test_gpu_warm_start_end_to_end():
+    """Train a small U-Net on SIMP OC trajectories of a 16x8x8 box (library solves and
+    kernels end to end), then solve held-out designs from zero and from the network's
+    warm start.  Both must reach the same solution on the true FP64 residual; the
+    iteration counts of both are printed: that difference is what the network gains."""
+    from platoanalyze_amd import workloads as wl
+    from platoanalyze_amd.elliptic import Problem
+    from platoanalyze_amd.warmstart import WarmStart, train
+
+    print("case: warm start 16x8x8, 4 trajectories x 6 steps", flush=True)
+    base = wl.make("cantilever_64x32x32", tiny=(16, 8, 8))
+    net, c0, hist, held = train(base, 4, 6, epochs=20, batch=4, lr=2e-3, base_channels=8, seed=3,
+                                log=lambda s: None)
+    assert hist[-1] < hist[0]  # the library's own kernels trained the network
+    cold, warm = [], []
+    for w, z, f in held[::2]:
+        m = wl.model(w)
+        pb = Problem(m, tolerance=1e-9, check_every=8, precision="mixed")
+        u_c = pb.solution(z)
+        cold.append(pb.stats.iterations)
+        u0 = WarmStart(net, w, m, c0).initial_guess(z, f)
+        u_w = pb.solution(z, u0=u0)
+        warm.append(pb.stats.iterations)
+        assert pb.stats.converged and pb.stats.rel_residual <= 1e-9
+        assert float((u_w - u_c).norm() / u_c.norm()) < 1e-6
+    print(f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; iterations cold {cold} warm {warm}", flush=True)
