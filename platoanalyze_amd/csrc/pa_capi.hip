@@ -318,9 +318,15 @@ int pa_finalize(pa_handle h) {
   pa::NodeElems adj = pa::node_elems(h->conn.data(), h->ne, h->nn);
   int max_blocks = kMaxChunkBlocks;
   if (const char* env = std::getenv("PA_CHUNK_BLOCKS")) max_blocks = std::max(64, std::min(4096, std::atoi(env)));
+  int max_rows = kMaxChunkRows, max_nodes = kMaxChunkNodes;  // tuning knobs, bounded by LDS below
+  if (const char* env = std::getenv("PA_CHUNK_ROWS")) max_rows = std::max(16, std::min(1024, std::atoi(env)));
+  if (const char* env = std::getenv("PA_CHUNK_NODES")) max_nodes = std::max(64, std::min(4096, std::atoi(env)));
   const int wpc = h->code_bits == 4 ? 2 : 4;
-  h->chunks = pa::build_chunks(g, adj, h->conn.data(), codes, wpc, max_blocks, kMaxChunkRows, kMaxChunkNodes);
-  if (h->chunks.max_blocks > max_blocks || h->chunks.max_nodes > kMaxChunkNodes)
+  h->chunks = pa::build_chunks(g, adj, h->conn.data(), codes, wpc, max_blocks, max_rows, max_nodes);
+  for (int mode = 0; mode <= pa::kModeAdjZ; ++mode)
+    if (pa::chunk_lds_bytes(mode, h->chunks.max_rows, h->chunks.max_blocks, h->chunks.max_nodes) > 160 * 1024)
+      return fail(PA_ERR_UNSUPPORTED, "chunk limits exceed the 160 KiB of LDS of a work-group");
+  if (h->chunks.max_blocks > max_blocks || h->chunks.max_nodes > max_nodes)
     return fail(PA_ERR_UNSUPPORTED, "a single graph row exceeds the LDS chunk budget");
   std::vector<int32_t> diag(h->nn);
   for (int r = 0; r < h->nn; ++r) {

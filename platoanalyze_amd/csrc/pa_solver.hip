@@ -203,8 +203,7 @@ __global__ __launch_bounds__(kT) void k_sum(const double* partials, int n, doubl
 // 3x3 block-diagonal inverse (cofactors)
 __global__ __launch_bounds__(kT) void k_diag_inv(const double* vals, const int* diag_slot, double* dinv,
                                                  int nnodes) {
-  const int n = blockIdx.x * kT + threadIdx.x;
-  if (n >= nnodes) return;
+  for (int n = blockIdx.x * kT + threadIdx.x; n < nnodes; n += gridDim.x * kT) {
   const double* a = vals + 9 * (size_t)diag_slot[n];
   const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
   const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
@@ -219,17 +218,18 @@ __global__ __launch_bounds__(kT) void k_diag_inv(const double* vals, const int* 
   o[6] = c02 * id;
   o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
   o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+  }
 }
 
 // ------------------------------------------------------------------ loads and BCs
 // y[3n+i] += s * f[3k+i] for the loaded nodes (sparse traction vector).
 __global__ __launch_bounds__(kT) void k_add_load(double* y, const int* nodes, const double* f, int n, double s) {
-  const int k = blockIdx.x * kT + threadIdx.x;
-  if (k >= n) return;
-  const size_t d = 3 * (size_t)nodes[k];
-  y[d] += s * f[3 * k];
-  y[d + 1] += s * f[3 * k + 1];
-  y[d + 2] += s * f[3 * k + 2];
+  for (int k = blockIdx.x * kT + threadIdx.x; k < n; k += gridDim.x * kT) {
+    const size_t d = 3 * (size_t)nodes[k];
+    y[d] += s * f[3 * k];
+    y[d + 1] += s * f[3 * k + 1];
+    y[d + 2] += s * f[3 * k + 2];
+  }
 }
 
 __global__ __launch_bounds__(kT) void k_fill(double* y, size_t n, double v) {
@@ -249,17 +249,17 @@ __global__ __launch_bounds__(kT) void k_scale(double* y, size_t n, double s) {
 __global__ __launch_bounds__(kT) void k_bc_lift(const double* vals, const int* rowmap, const int* col,
                                                 const int* dofs, const double* bcv, int nbc, double scale,
                                                 double* rhs) {
-  const int k = blockIdx.x * kT + threadIdx.x;
-  if (k >= nbc) return;
-  const double v = scale * bcv[k];
-  if (v == 0.0) return;
-  const int d = dofs[k], n = d / 3, i = d % 3;
-  for (int s = rowmap[n]; s < rowmap[n + 1]; ++s) {
-    const int c = col[s];
-    for (int j = 0; j < 3; ++j) {
-      const int cd = 3 * c + j;
-      if (cd == d) continue;
-      atomicAdd(&rhs[cd], -vals[9 * (size_t)s + 3 * i + j] * v);
+  for (int k = blockIdx.x * kT + threadIdx.x; k < nbc; k += gridDim.x * kT) {
+    const double v = scale * bcv[k];
+    if (v == 0.0) continue;
+    const int d = dofs[k], n = d / 3, i = d % 3;
+    for (int s = rowmap[n]; s < rowmap[n + 1]; ++s) {
+      const int c = col[s];
+      for (int j = 0; j < 3; ++j) {
+        const int cd = 3 * c + j;
+        if (cd == d) continue;
+        atomicAdd(&rhs[cd], -vals[9 * (size_t)s + 3 * i + j] * v);
+      }
     }
   }
 }
@@ -268,17 +268,17 @@ __global__ __launch_bounds__(kT) void k_bc_lift(const double* vals, const int* r
 // elsewhere; the symmetric (col,row) entries zeroed too.
 __global__ __launch_bounds__(kT) void k_bc_zero(double* vals, const int* rowmap, const int* col,
                                                 const int* dofs, int nbc) {
-  const int k = blockIdx.x * kT + threadIdx.x;
-  if (k >= nbc) return;
-  const int d = dofs[k], n = d / 3, i = d % 3;
-  for (int s = rowmap[n]; s < rowmap[n + 1]; ++s) {
-    const int c = col[s];
-    for (int j = 0; j < 3; ++j) {
-      const int cd = 3 * c + j;
-      vals[9 * (size_t)s + 3 * i + j] = (cd == d) ? 1.0 : 0.0;
-      if (cd == d) continue;
-      for (int t = rowmap[c]; t < rowmap[c + 1]; ++t)
-        if (col[t] == n) vals[9 * (size_t)t + 3 * j + i] = 0.0;
+  for (int k = blockIdx.x * kT + threadIdx.x; k < nbc; k += gridDim.x * kT) {
+    const int d = dofs[k], n = d / 3, i = d % 3;
+    for (int s = rowmap[n]; s < rowmap[n + 1]; ++s) {
+      const int c = col[s];
+      for (int j = 0; j < 3; ++j) {
+        const int cd = 3 * c + j;
+        vals[9 * (size_t)s + 3 * i + j] = (cd == d) ? 1.0 : 0.0;
+        if (cd == d) continue;
+        for (int t = rowmap[c]; t < rowmap[c + 1]; ++t)
+          if (col[t] == n) vals[9 * (size_t)t + 3 * j + i] = 0.0;
+      }
     }
   }
 }
@@ -286,16 +286,12 @@ __global__ __launch_bounds__(kT) void k_bc_zero(double* vals, const int* rowmap,
 // second loop: rhs(dof) = scale * value
 __global__ __launch_bounds__(kT) void k_bc_set(double* rhs, const int* dofs, const double* bcv, int nbc,
                                                double scale) {
-  const int k = blockIdx.x * kT + threadIdx.x;
-  if (k >= nbc) return;
-  rhs[dofs[k]] = scale * bcv[k];
+  for (int k = blockIdx.x * kT + threadIdx.x; k < nbc; k += gridDim.x * kT) rhs[dofs[k]] = scale * bcv[k];
 }
 
 // zero the BC dofs of a vector (homogeneous constraint on a correction)
 __global__ __launch_bounds__(kT) void k_bc_zero_vec(double* y, const int* dofs, int nbc) {
-  const int k = blockIdx.x * kT + threadIdx.x;
-  if (k >= nbc) return;
-  y[dofs[k]] = 0.0;
+  for (int k = blockIdx.x * kT + threadIdx.x; k < nbc; k += gridDim.x * kT) y[dofs[k]] = 0.0;
 }
 
 }  // namespace pa
