@@ -69,7 +69,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for scaling HBM counters)")
     ap.add_argument("--profile-solve", type=int, default=0, metavar="K",
-                    help="run only the FP64 and mixed solves, each capped at K iterations (for profilers)")
+                    help="run only the solves (FP64/mixed x Jacobi/multigrid), each capped at K iterations "
+                         "(for profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the assembly/solve path runs on a single MI355X "
@@ -98,8 +99,9 @@ def main() -> int:
         m = wl.model(w, device=dev)
         z = torch.as_tensor(w.z, device=dev)
         out = {}
-        for prec in ("fp64", "mixed"):
-            pb = Problem(m, tolerance=1e-8, check_every=32, precision=prec, max_iters=args.profile_solve)
+        for prec, pre in (("fp64", "jacobi"), ("mixed", "jacobi"), ("fp64", "multigrid"), ("mixed", "multigrid")):
+            pb = Problem(m, tolerance=1e-8, check_every=32, precision=prec, preconditioner=pre,
+                         max_iters=args.profile_solve)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             try:
@@ -108,7 +110,7 @@ def main() -> int:
                 if e.status != 5:
                     raise
             torch.cuda.synchronize()
-            out[prec] = {"seconds": time.perf_counter() - t0, "iterations": pb.stats.iterations}
+            out[f"{prec}_{pre}"] = {"seconds": time.perf_counter() - t0, "iterations": pb.stats.iterations}
         print(json.dumps({"profile_solve": out, "config": args.config}))
         return 0
     t0 = time.perf_counter()
@@ -158,31 +160,46 @@ def main() -> int:
 
     extra = {}
     if not args.no_solve:
-        # solve K(z)u = f to 1e-8 relative residual, then C and dC/dz; FP64 first, then
-        # FP64 PCG with the FP32 matrix-free operator and true-residual replacement,
-        # compared against it
+        # solve K(z)u = f to 1e-8 relative residual, then C and dC/dz, with each solver:
+        # FP64 block-CRS PCG and FP64 PCG with the FP32 matrix-free operator and
+        # true-residual replacement, each with 3x3 block-Jacobi or geometric multigrid
+        # preconditioning.  Each variant runs twice: the first call builds the multigrid
+        # hierarchy (once per mesh), the second is timed; errors are against FP64 Jacobi.
+        methods = {
+            ("fp64", "jacobi"): "FP64 block-CRS PCG, 3x3 block Jacobi",
+            ("mixed", "jacobi"): "FP64 PCG, FP32 matrix-free operator, true-residual replacement, 3x3 block Jacobi",
+            ("fp64", "multigrid"): "FP64 block-CRS PCG, geometric multigrid V-cycle",
+            ("mixed", "multigrid"): "FP64 PCG, FP32 matrix-free operator, true-residual replacement, "
+                                    "geometric multigrid V-cycle",
+        }
         res = {}
-        for prec in ("fp64", "mixed"):
-            pb = Problem(m, tolerance=1e-8, check_every=32, precision=prec)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            u_s = pb.solution(z)
-            g, Cval = pb.criterionGradient(z, u_s)
-            torch.cuda.synchronize()
-            res[prec] = {"seconds": time.perf_counter() - t0, "iterations": pb.stats.iterations,
-                         "outer_iterations": pb.stats.outer_iterations,
-                         "true_rel_residual": pb.stats.rel_residual, "compliance": Cval, "u": u_s, "g": g}
-        a, b = res["fp64"], res["mixed"]
+        for (prec, pre) in methods:
+            pb = Problem(m, tolerance=1e-8, check_every=32 if pre == "jacobi" else 4, precision=prec,
+                         preconditioner=pre)
+            secs = []
+            for _ in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                u_s = pb.solution(z)
+                g, Cval = pb.criterionGradient(z, u_s)
+                torch.cuda.synchronize()
+                secs.append(time.perf_counter() - t0)
+            res[(prec, pre)] = {"seconds": secs[1], "first_call_seconds": secs[0],
+                                "iterations": pb.stats.iterations, "outer_iterations": pb.stats.outer_iterations,
+                                "true_rel_residual": pb.stats.rel_residual, "compliance": Cval, "u": u_s, "g": g}
+        a = res[("fp64", "jacobi")]
         extra["solve_sensitivity_s"] = a["seconds"]
-        extra["solve"] = {
-            "fp64": {"method": "FP64 block-CRS 3x3-block-Jacobi PCG", **{k: a[k] for k in
-                     ("seconds", "iterations", "true_rel_residual", "compliance")}},
-            "mixed": {"method": "FP64 Jacobi-PCG, FP32 matrix-free operator, true-residual replacement", **{k: b[k] for k in
-                      ("seconds", "iterations", "outer_iterations", "true_rel_residual", "compliance")},
-                      "rel_err_u": float((b["u"] - a["u"]).norm() / a["u"].norm()),
-                      "rel_err_compliance": abs(b["compliance"] - a["compliance"]) / abs(a["compliance"]),
-                      "rel_err_gradient": float((b["g"] - a["g"]).norm() / a["g"].norm())},
-            "speedup_mixed_vs_fp64": a["seconds"] / b["seconds"]}
+        solve_out = {}
+        for key, r in res.items():
+            d = {"method": methods[key], **{k: r[k] for k in ("seconds", "first_call_seconds", "iterations",
+                                                              "outer_iterations", "true_rel_residual", "compliance")}}
+            if key != ("fp64", "jacobi"):
+                d.update({"rel_err_u": float((r["u"] - a["u"]).norm() / a["u"].norm()),
+                          "rel_err_compliance": abs(r["compliance"] - a["compliance"]) / abs(a["compliance"]),
+                          "rel_err_gradient": float((r["g"] - a["g"]).norm() / a["g"].norm()),
+                          "speedup_vs_fp64_jacobi": a["seconds"] / r["seconds"]})
+            solve_out[f"{key[0]}_{key[1]}"] = d
+        extra["solve"] = solve_out
 
     cpu = None
     if not args.no_cpu:

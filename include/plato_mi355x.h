@@ -131,11 +131,13 @@ int pa_sf_gradient_z(pa_handle h, const double* u, const double* z, double* g, d
 typedef struct {
   double tolerance;   /* relative residual ||b-Ax||/||b|| to stop at                 */
   int max_iters;      /* iteration cap (PA_ERR_NOT_CONVERGED when reached)           */
-  int check_every;    /* host convergence check period (iterations)                  */
+  int check_every;    /* host convergence check period (iterations; multigrid: <= 2) */
   int precision;      /* 0: FP64 block-CRS PCG; 1: FP64 PCG with an FP32 matrix-free operator */
   int use_x0;         /* 1: start from the contents of x (warm start), 0: from zero  */
   double inner_tolerance; /* precision 1: drop of the iterated residual between true FP64
                              residual replacements (clamped to [1e-6, 0.5]; 0 -> 0.1)     */
+  int preconditioner;     /* 0: 3x3 block Jacobi; 1: geometric multigrid V-cycle (needs
+                             pa_set_box; pa_solution and pa_stress_pnorm_gradient only)  */
 } pa_solver_opts;
 
 typedef struct {
@@ -164,6 +166,16 @@ int pa_solve(pa_handle h, const double* values, double* x, const double* b,
  * (warm start; its constrained dofs are reset to their prescribed values). */
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
+
+/* Declares the mesh a structured box of nx*ny*nz hexes (6 Tet4 each) whose nodes lie on
+ * the (nx+1)(ny+1)(nz+1) lattice spanning their bounding box, in any order and
+ * numbering.  Enables preconditioner 1: a geometric multigrid hierarchy of boxes with
+ * half the hexes per axis while the counts stay even, rediscretised with densities
+ * homogenised through the SIMP stiffness weight, Chebyshev block-Jacobi smoothing and a
+ * dense solve on the last level.  No counterpart in the reference, whose solver
+ * interface defers to external algebraic multigrid (src/alg/AmgXLinearSolver.hpp,
+ * src/alg/EpetraLinearSolver.cpp); PA_ERR_MESH when the nodes are not such a lattice. */
+int pa_set_box(pa_handle h, int nx, int ny, int nz);
 
 /* Damped block-Jacobi smoothing of the homogeneous problem, u <- u - omega D^-1 K(z) u
  * at the free dofs, `steps` times (D: 3x3 diagonal blocks of K).  A low-pass filter

@@ -26,7 +26,8 @@ class PaError(RuntimeError):
 
 class SolverOpts(C.Structure):
     _fields_ = [("tolerance", C.c_double), ("max_iters", C.c_int), ("check_every", C.c_int),
-                ("precision", C.c_int), ("use_x0", C.c_int), ("inner_tolerance", C.c_double)]
+                ("precision", C.c_int), ("use_x0", C.c_int), ("inner_tolerance", C.c_double),
+                ("preconditioner", C.c_int)]
 
 
 class SolverStats(C.Structure):
@@ -73,6 +74,7 @@ _SIGS = {
     "pa_stress_pnorm_gradient": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _PD, C.POINTER(SolverOpts),
                                       C.POINTER(SolverStats)]),
     "pa_smooth": (_I, [_P, _P, _P, _I, _D]),
+    "pa_set_box": (_I, [_P, _I, _I, _I]),
     "pa_unet_last_error": (C.c_char_p, []),
     "pa_unet_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, C.c_ulonglong]),
     "pa_unet_destroy": (_I, [_P]),

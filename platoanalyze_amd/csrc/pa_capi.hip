@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -13,6 +14,7 @@
 #include "pa_host.hpp"
 #include "pa_kernels.hpp"
 #include "pa_mf.hpp"
+#include "pa_mg.hpp"
 #include "pa_reduce.hpp"
 #include "pa_solver.hpp"
 
@@ -61,6 +63,21 @@ struct DevArray {
 
 }  // namespace
 
+// One level of the geometric multigrid hierarchy of a box mesh.  Level 0 is the
+// problem itself; coarser levels own a problem handle on the coarsened box.
+struct MgLevel {
+  pa_handle h = nullptr;  // owned for levels >= 1
+  bool owned = false;
+  pa::BoxDims dims{0, 0, 0};
+  int nn = 0;
+  DevArray<int32_t> grid;          // (i, j, k) per node
+  DevArray<int32_t> node_of_grid;  // node per grid point
+  DevArray<double> r, x, y, d, t, rr, w, zc, vals, dinv;
+  DevArray<double> pw;             // power-iteration vector
+  double inv_theta = 0.0, c1 = 0.0, c2 = 0.0, lam = 0.0;
+  ~MgLevel();
+};
+
 struct pa_problem {
   int nn = 0, ne = 0;
   std::vector<double> coords;
@@ -103,6 +120,15 @@ struct pa_problem {
   DevArray<double> chunk_partials;
   // solver workspace (lazily allocated)
   DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
+
+  // geometric multigrid (box meshes, pa_set_box)
+  bool has_box = false;
+  pa::BoxDims box{0, 0, 0};       // node counts per axis
+  double box_lo[3] = {0, 0, 0}, box_len[3] = {0, 0, 0};
+  std::vector<int32_t> box_grid;  // (i, j, k) per node, host
+  std::vector<std::unique_ptr<MgLevel>> mg;
+  DevArray<double> mg_Ainv;
+  DevArray<double> g_x, g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -147,6 +173,10 @@ struct pa_problem {
     if (host_scal) (void)hipHostFree(host_scal);
   }
 };
+
+MgLevel::~MgLevel() {
+  if (owned) delete h;
+}
 
 static int grid_for(size_t n, int per_block = 256, int cap = 4096) {
   size_t g = (n + per_block - 1) / per_block;
@@ -522,6 +552,8 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   if (!vals || !x || !b) return fail(PA_ERR_ARG, "pa_solve: null pointer");
   pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
+  if (opts.preconditioner != 0)
+    return fail(PA_ERR_ARG, "pa_solve: only block Jacobi (multigrid needs the density: pa_solution)");
   if (int rc = ensure_solver_ws(h)) return rc;
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
@@ -691,9 +723,11 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, pa::kBetaInit, s));
   }
   int it = 0;
+  bool restart = true;  // the next apply starts the direction from p = z
   while (!conv) {
     const int cur = it & 1;
-    ma.first = it == 0;
+    ma.first = restart ? 1 : 0;
+    restart = false;
     ma.p_prev = pbuf[cur ^ 1];
     ma.p_cur = pbuf[cur];
     PA_HIP(pa::launch_mf_apply(ma, s));
@@ -721,9 +755,12 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
       break;
     }
     if (last) break;
-    // continue from the replaced residual: z = Dinv r, beta = (r.z) / rz of the last step
+    // continue from the replaced residual: z = Dinv r, beta = (r.z) / rz of the last step;
+    // restart from p = z instead when the true residual is well above the iterated one
+    restart = rr_ref > 4.0 * rr_it;
     PA_HIP(pa::launch_pcgm_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
-    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, pa::kBetaReplace, s));
+    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, restart ? pa::kBetaInit : pa::kBetaReplace,
+                                s));
   }
   if (st) {
     st->iterations = inner_total;
@@ -733,6 +770,481 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   }
   if (!conv)
     return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve stopped at relative residual " + std::to_string(rel));
+  return PA_OK;
+}
+
+// ---------------------------------------------------------------- geometric multigrid
+// Grid coordinates (i, j, k) of every node of a box node grid with `d` nodes per axis
+// over [lo, lo + len]; false when a node is off the lattice or two nodes share a point.
+static bool box_grid(const double* X, int nn, const pa::BoxDims& d, const double lo[3], const double len[3],
+                     std::vector<int32_t>& grid) {
+  grid.assign(3 * (size_t)nn, 0);
+  std::vector<uint8_t> seen((size_t)d.nx * d.ny * d.nz, 0);
+  const int n3[3] = {d.nx, d.ny, d.nz};
+  for (int n = 0; n < nn; ++n) {
+    for (int a = 0; a < 3; ++a) {
+      const double hstep = len[a] / (n3[a] - 1);
+      const double t = (X[3 * (size_t)n + a] - lo[a]) / hstep;
+      const double r = std::nearbyint(t);
+      if (std::fabs(t - r) > 1e-6 || r < 0 || r > n3[a] - 1) return false;
+      grid[3 * (size_t)n + a] = (int32_t)r;
+    }
+    const size_t g = ((size_t)grid[3 * (size_t)n] * d.ny + grid[3 * (size_t)n + 1]) * d.nz + grid[3 * (size_t)n + 2];
+    if (seen[g]) return false;
+    seen[g] = 1;
+  }
+  return true;
+}
+
+int pa_set_box(pa_handle h, int nx, int ny, int nz) {
+  if (!h) return fail(PA_ERR_ARG, "null handle");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(PA_ERR_ARG, "pa_set_box: hex counts must be >= 1");
+  if ((size_t)(nx + 1) * (ny + 1) * (nz + 1) != (size_t)h->nn)
+    return fail(PA_ERR_MESH, "pa_set_box: node count is not (nx+1)(ny+1)(nz+1)");
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int n = 0; n < h->nn; ++n)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], h->coords[3 * (size_t)n + a]);
+      hi[a] = std::max(hi[a], h->coords[3 * (size_t)n + a]);
+    }
+  const pa::BoxDims d{nx + 1, ny + 1, nz + 1};
+  double len[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  std::vector<int32_t> grid;
+  if (!box_grid(h->coords.data(), h->nn, d, lo, len, grid))
+    return fail(PA_ERR_MESH, "pa_set_box: the nodes do not form the box's node lattice");
+  h->box = d;
+  for (int a = 0; a < 3; ++a) {
+    h->box_lo[a] = lo[a];
+    h->box_len[a] = len[a];
+  }
+  h->box_grid.swap(grid);
+  h->has_box = true;
+  h->mg.clear();
+  return PA_OK;
+}
+
+static int mg_level_init(pa_handle h, MgLevel& L, const std::vector<int32_t>& grid) {
+  (void)h;
+  PA_HIP(L.grid.upload(grid));
+  std::vector<int32_t> nog((size_t)L.dims.nx * L.dims.ny * L.dims.nz, 0);
+  for (int n = 0; n < L.nn; ++n)
+    nog[((size_t)grid[3 * (size_t)n] * L.dims.ny + grid[3 * (size_t)n + 1]) * L.dims.nz + grid[3 * (size_t)n + 2]] = n;
+  PA_HIP(L.node_of_grid.upload(nog));
+  const size_t nd = 3 * (size_t)L.nn;
+  PA_HIP(L.r.alloc(nd));
+  PA_HIP(L.x.alloc(nd));
+  PA_HIP(L.y.alloc(nd));
+  PA_HIP(L.d.alloc(nd));
+  PA_HIP(L.t.alloc(nd));
+  PA_HIP(L.rr.alloc(nd));
+  PA_HIP(L.pw.alloc(nd));
+  PA_HIP(L.w.alloc(L.nn));
+  PA_HIP(L.zc.alloc(L.nn));
+  PA_HIP(L.dinv.alloc(9 * (size_t)L.nn));
+  return PA_OK;
+}
+
+// Levels: the problem itself, then boxes with half the hexes per axis while every count
+// stays even and the level has more than 64 nodes; the last level is solved densely.
+static int mg_build(pa_handle h) {
+  if (!h->mg.empty()) return PA_OK;
+  if (!h->has_box) return fail(PA_ERR_STATE, "multigrid needs the box grid (pa_set_box)");
+  auto L0 = std::make_unique<MgLevel>();
+  L0->h = h;
+  L0->dims = h->box;
+  L0->nn = h->nn;
+  if (int rc = mg_level_init(h, *L0, h->box_grid)) return rc;
+  std::vector<int32_t> prev_grid = h->box_grid;
+  h->mg.push_back(std::move(L0));
+  while (true) {
+    MgLevel& P = *h->mg.back();
+    const int hx = P.dims.nx - 1, hy = P.dims.ny - 1, hz = P.dims.nz - 1;
+    if (P.nn <= 64 || hx % 2 || hy % 2 || hz % 2) break;
+    pa::BoxMesh bm = pa::build_box(hx / 2, hy / 2, hz / 2, h->box_len[0], h->box_len[1], h->box_len[2]);
+    for (int n = 0; n < bm.num_nodes; ++n)
+      for (int a = 0; a < 3; ++a) bm.coords[3 * (size_t)n + a] += h->box_lo[a];
+    auto L = std::make_unique<MgLevel>();
+    L->dims = pa::BoxDims{hx / 2 + 1, hy / 2 + 1, hz / 2 + 1};
+    L->nn = bm.num_nodes;
+    std::vector<int32_t> grid;
+    if ((size_t)bm.num_nodes != (size_t)L->dims.nx * L->dims.ny * L->dims.nz ||
+        !box_grid(bm.coords.data(), bm.num_nodes, L->dims, h->box_lo, h->box_len, grid))
+      return fail(PA_ERR_MESH, "multigrid: coarse box does not form its lattice");
+    pa_handle hc = nullptr;
+    if (int rc = pa_create(&hc, bm.num_nodes, bm.num_elems, bm.coords.data(), bm.conn.data())) return rc;
+    L->h = hc;
+    L->owned = true;
+    if (int rc = pa_set_material(hc, h->E, h->nu)) return rc;
+    if (int rc = pa_set_penalty(hc, h->p, h->rmin)) return rc;
+    // a coarse dof is constrained when the coincident node of the finer level is
+    std::vector<uint8_t> pmask(P.nn, 0);
+    for (int32_t dof : P.h->bc_dofs) pmask[dof / 3] |= (uint8_t)(1u << (dof % 3));
+    std::vector<int32_t> pnog((size_t)P.dims.nx * P.dims.ny * P.dims.nz, 0);
+    for (int n = 0; n < P.nn; ++n)
+      pnog[((size_t)prev_grid[3 * (size_t)n] * P.dims.ny + prev_grid[3 * (size_t)n + 1]) * P.dims.nz +
+           prev_grid[3 * (size_t)n + 2]] = n;
+    for (int comp = 0; comp < 3; ++comp) {
+      std::vector<int32_t> nodes;
+      for (int n = 0; n < L->nn; ++n) {
+        const size_t pg = ((size_t)(2 * grid[3 * (size_t)n]) * P.dims.ny + 2 * grid[3 * (size_t)n + 1]) * P.dims.nz +
+                          2 * grid[3 * (size_t)n + 2];
+        if ((pmask[pnog[pg]] >> comp) & 1u) nodes.push_back(n);
+      }
+      if (!nodes.empty())
+        if (int rc = pa_add_essential_bc(hc, (int)nodes.size(), nodes.data(), comp, 0.0)) return rc;
+    }
+    if (int rc = pa_finalize(hc)) return rc;
+    hc->stream = h->stream;
+    if (int rc = mg_level_init(h, *L, grid)) return rc;
+    PA_HIP(L->vals.alloc(9 * (size_t)hc->graph.rowmap[hc->nn]));
+    prev_grid.swap(grid);
+    h->mg.push_back(std::move(L));
+  }
+  if (h->mg.size() < 2) return fail(PA_ERR_UNSUPPORTED, "multigrid: the box has no coarser level");
+  const size_t nc = 3 * (size_t)h->mg.back()->nn;
+  PA_HIP(h->mg_Ainv.alloc(nc * nc));
+  return PA_OK;
+}
+
+// y = K_l x: the fine operator (FP64 CRS `vals0`, or the FP32 matrix-free apply) on
+// level 0, the assembled coarse CRS below.
+static int mg_apply(pa_handle h, size_t l, const double* x, double* y, const double* vals0, bool mixed) {
+  hipStream_t s = h->stream;
+  if (l == 0) {
+    if (mixed) {
+      pa::MfArgs ma{};
+      ma.chunk_rows = h->chunk_rows.p;
+      ma.node_offs = h->chunk_node_offs.p;
+      ma.nodes = h->chunk_nodes.p;
+      ma.ent_ln = h->ent_ln.p;
+      ma.list_offs = h->list_offs.p;
+      ma.nchunks = h->chunks.count;
+      ma.max_rows = h->chunks.max_rows;
+      ma.max_nodes = h->chunks.max_nodes;
+      ma.Xz = h->m_xz.p;
+      ma.bcmask = h->bcmask.p;
+      ma.z = x;
+      ma.p_prev = nullptr;
+      ma.p_cur = h->g_pc.p;
+      ma.first = 1;
+      ma.scal = h->scal.p;
+      ma.y = y;
+      ma.partials = h->chunk_partials.p;
+      ma.m = h->material();
+      PA_HIP(pa::launch_mf_apply(ma, s));
+    } else {
+      pa::k_spmv<<<pa::red_grid((size_t)h->nn, 16), 256, 0, s>>>(vals0, h->rowmap.p, h->colidx.p, x, nullptr, y,
+                                                                  h->nn, 0);
+    }
+  } else {
+    MgLevel& L = *h->mg[l];
+    pa::k_spmv<<<pa::red_grid((size_t)L.nn, 16), 256, 0, s>>>(L.vals.p, L.h->rowmap.p, L.h->colidx.p, x, nullptr,
+                                                                y, L.nn, 0);
+  }
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+static const double* mg_dinv(pa_handle h, size_t l, bool mixed) {
+  if (l > 0) return h->mg[l]->dinv.p;
+  return mixed ? h->m_dinv.p : h->w_dinv.p;
+}
+
+// x = V(r): symmetric V-cycle, degree-2 Chebyshev block-Jacobi pre- and post-smoothing,
+// full-weighting restriction, trilinear prolongation, dense solve on the last level.
+static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const double* vals0, bool mixed) {
+  hipStream_t s = h->stream;
+  MgLevel& L = *h->mg[l];
+  const size_t nd = 3 * (size_t)L.nn;
+  if (l + 1 == h->mg.size()) {
+    PA_HIP(pa::launch_dense_gemv(h->mg_Ainv.p, r, x, (int)nd, s));
+    return PA_OK;
+  }
+  MgLevel& C = *h->mg[l + 1];
+  const double* dinv = mg_dinv(h, l, mixed);
+  // pre-smoothing from x = 0
+  PA_HIP(pa::launch_cheb_first(x, L.d.p, r, dinv, L.inv_theta, L.nn, s));
+  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+  PA_HIP(pa::launch_cheb_next(x, L.d.p, r, L.t.p, dinv, L.c1, L.c2, L.nn, s));
+  // coarse-grid correction
+  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+  PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
+  PA_HIP(pa::launch_mg_restrict(L.rr.p, C.r.p, C.grid.p, L.node_of_grid.p, L.dims, C.h->bcmask.p, C.nn, s));
+  if (int rc = mg_vcycle(h, l + 1, C.r.p, C.x.p, vals0, mixed)) return rc;
+  PA_HIP(pa::launch_mg_prolong_add(C.x.p, x, L.grid.p, C.node_of_grid.p, C.dims, L.h->bcmask.p, L.nn, s));
+  // post-smoothing on the remaining residual
+  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+  PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
+  PA_HIP(pa::launch_cheb_first(L.y.p, L.d.p, L.rr.p, dinv, L.inv_theta, L.nn, s));
+  if (int rc = mg_apply(h, l, L.y.p, L.t.p, vals0, mixed)) return rc;
+  PA_HIP(pa::launch_cheb_next(L.y.p, L.d.p, L.rr.p, L.t.p, dinv, L.c1, L.c2, L.nn, s));
+  PA_HIP(pa::launch_add(x, L.y.p, nd, s));
+  return PA_OK;
+}
+
+// sum of per-group partials of a.b (n3 values) into scal[slot], read back
+static int mg_dot(pa_handle h, const double* a, const double* b, size_t n, int slot, double* out) {
+  double* P = h->partials.p + (size_t)pa::kPDot * pa::kRedBlocks;
+  const int G = pa::red_grid(n / 3 + 1, 256);
+  pa::k_dot<<<G, 256, 0, h->stream>>>(a, b, n, P);
+  pa::k_sum<<<1, 256, 0, h->stream>>>(P, G, h->scal.p, slot);
+  PA_HIP(hipGetLastError());
+  if (out) {
+    if (int rc = read_scal(h)) return rc;
+    *out = h->host_scal[slot];
+  }
+  return PA_OK;
+}
+
+// PA_MG_TRACE=1: per-level spectral estimates and the convergence checks on stderr
+static bool mg_trace() {
+  static const bool on = std::getenv("PA_MG_TRACE") != nullptr;
+  return on;
+}
+
+// Inverse of a symmetric positive definite n x n matrix (row major, in place in A;
+// U is n x n scratch): Cholesky A = L L^T into the lower triangle, U = L^-T (upper),
+// A^-1 = U U^T.  Every inner loop runs along contiguous rows.
+static int spd_inverse(std::vector<double>& A, std::vector<double>& U, int n) {
+  for (int j = 0; j < n; ++j) {
+    double* Aj = &A[(size_t)j * n];
+    double d = Aj[j];
+    for (int k = 0; k < j; ++k) d -= Aj[k] * Aj[k];
+    if (!(d > 0.0)) return fail(PA_ERR_NOT_CONVERGED, "multigrid: coarsest operator is not positive definite");
+    const double ljj = std::sqrt(d), inv = 1.0 / ljj;
+    Aj[j] = ljj;
+    for (int i = j + 1; i < n; ++i) {
+      double* Ai = &A[(size_t)i * n];
+      double t = Ai[j];
+      for (int k = 0; k < j; ++k) t -= Ai[k] * Aj[k];
+      Ai[j] = t * inv;
+    }
+  }
+  // row j of U holds column j of L^-1: U[j][i] = (L^-1)[i][j], i >= j
+  for (int j = 0; j < n; ++j) {
+    double* Uj = &U[(size_t)j * n];
+    Uj[j] = 1.0 / A[(size_t)j * n + j];
+    for (int i = j + 1; i < n; ++i) {
+      const double* Li = &A[(size_t)i * n];
+      double t = 0.0;
+      for (int k = j; k < i; ++k) t += Li[k] * Uj[k];
+      Uj[i] = -t / Li[i];
+    }
+  }
+  // A^-1[i][j] = sum_{k >= max(i, j)} (L^-1)[k][i] (L^-1)[k][j] = sum_k U[i][k] U[j][k]
+  for (int i = 0; i < n; ++i) {
+    const double* Ui = &U[(size_t)i * n];
+    for (int j = i; j < n; ++j) {
+      const double* Uj = &U[(size_t)j * n];
+      double t = 0.0;
+      for (int k = j; k < n; ++k) t += Ui[k] * Uj[k];
+      A[(size_t)i * n + j] = t;
+      A[(size_t)j * n + i] = t;
+    }
+  }
+  return PA_OK;
+}
+
+// Per solve: homogenised coarse densities (restricted SIMP weights), coarse Jacobians,
+// their block diagonals, Chebyshev intervals by power iteration, coarsest inverse.
+static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixed) {
+  hipStream_t s = h->stream;
+  const size_t nl = h->mg.size();
+  PA_HIP(pa::launch_simp_w(z, h->mg[0]->w.p, h->nn, h->rmin, h->p, s));
+  for (size_t l = 1; l < nl; ++l) {
+    MgLevel& F = *h->mg[l - 1];
+    MgLevel& L = *h->mg[l];
+    PA_HIP(pa::launch_mg_wrestrict(F.w.p, L.w.p, L.grid.p, F.node_of_grid.p, F.dims, L.nn, s));
+    PA_HIP(pa::launch_simp_z(L.w.p, L.zc.p, L.nn, h->rmin, h->p, s));
+    if (int rc = run_chunk(L.h, pa::kModeJacobian, nullptr, L.zc.p, L.vals.p, nullptr)) return rc;
+    if (int rc = pa_apply_constraints(L.h, L.vals.p, L.rr.p, 0.0)) return rc;
+    pa::k_diag_inv<<<grid_for(L.nn), 256, 0, s>>>(L.vals.p, L.h->diag_slot.p, L.dinv.p, L.nn);
+    PA_HIP(hipGetLastError());
+  }
+  // Chebyshev interval [lmax / 30, 1.1 lmax] of D^-1 K per smoothed level
+  for (size_t l = 0; l + 1 < nl; ++l) {
+    MgLevel& L = *h->mg[l];
+    const size_t nd = 3 * (size_t)L.nn;
+    const double* dinv = mg_dinv(h, l, mixed);
+    std::vector<double> v0(nd);
+    for (size_t i = 0; i < nd; ++i) v0[i] = (double)((i * 2654435761ull >> 7) % 2001) / 1000.0 - 1.0;
+    PA_HIP(hipMemcpyAsync(L.pw.p, v0.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
+    double nrm = 0.0;
+    if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, &nrm)) return rc;
+    double lam = 0.0;
+    for (int it = 0; it < 12; ++it) {
+      pa::k_scale<<<grid_for(nd), 256, 0, s>>>(L.pw.p, nd, 1.0 / std::sqrt(nrm));
+      if (int rc = mg_apply(h, l, L.pw.p, L.t.p, vals0, mixed)) return rc;
+      PA_HIP(pa::launch_dinv_apply(L.pw.p, dinv, L.t.p, L.nn, s));
+      if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, &nrm)) return rc;
+      lam = std::sqrt(nrm);
+    }
+    if (!(lam > 0.0) || !std::isfinite(lam)) return fail(PA_ERR_NOT_CONVERGED, "multigrid: bad spectral estimate");
+    const double lmax = 1.1 * lam, lmin = lmax / 30.0;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+    L.lam = lam;
+    if (mg_trace()) std::fprintf(stderr, "mg level %zu: %d nodes, lambda_max(D^-1 K) ~ %.6g\n", l, L.nn, lam);
+    L.inv_theta = 1.0 / theta;
+    L.c1 = rho1 * rho0;
+    L.c2 = 2.0 * rho1 / delta;
+  }
+  // coarsest level: dense inverse on the host (a few hundred dofs)
+  MgLevel& Lc = *h->mg.back();
+  const int nc = 3 * Lc.nn;
+  const pa::Graph& g = Lc.h->graph;
+  std::vector<double> v(9 * (size_t)g.rowmap[Lc.nn]);
+  PA_HIP(hipMemcpyAsync(v.data(), Lc.vals.p, v.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PA_HIP(hipStreamSynchronize(s));
+  std::vector<double> A((size_t)nc * nc, 0.0), U((size_t)nc * nc, 0.0);
+  for (int r = 0; r < Lc.nn; ++r)
+    for (int k = g.rowmap[r]; k < g.rowmap[r + 1]; ++k)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[(size_t)(3 * r + i) * nc + 3 * g.colidx[k] + j] = v[9 * (size_t)k + 3 * i + j];
+  if (int rc = spd_inverse(A, U, nc)) return rc;
+  PA_HIP(hipMemcpyAsync(h->mg_Ainv.p, A.data(), A.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  PA_HIP(hipStreamSynchronize(s));
+  return PA_OK;
+}
+
+// Multigrid-preconditioned CG.  FP64 path (mixed = false): the assembled, constrained
+// K in `vals0`, right-hand side b, x from zero (or x0).  Mixed path: the FP32
+// matrix-free operator on level 0, x = u with its prescribed values, right-hand side the
+// traction load (b == nullptr) or b (homogeneous constraints), and the iterated residual
+// replaced by the true FP64 residual every `inner_tolerance` decrease.
+static int solve_mg(pa_handle h, const double* z, double* x, const double* b, const double* vals0,
+                    pa_solver_opts opts, pa_solver_stats* st, bool mixed) {
+  if (int rc = mg_build(h)) return rc;
+  const int nn = h->nn;
+  const size_t ndof = 3 * (size_t)nn;
+  hipStream_t s = h->stream;
+  for (auto& L : h->mg)
+    if (L->owned) L->h->stream = s;
+  if (int rc = ensure_solver_ws(h)) return rc;
+  if (h->g_r.n != ndof) {
+    PA_HIP(h->g_r.alloc(ndof));
+    PA_HIP(h->g_z.alloc(ndof));
+    PA_HIP(h->g_p.alloc(ndof));
+    PA_HIP(h->g_q.alloc(ndof));
+    PA_HIP(h->g_pc.alloc(ndof));
+  }
+  const int G = pa::red_grid((size_t)nn, 256);
+  double* P = h->partials.p;
+  auto PS = [&](int slot) { return P + (size_t)slot * pa::kRedBlocks; };
+  double bb = 0.0, rr0 = 0.0;
+  const double delta = std::min(0.5, std::max(1e-6, opts.inner_tolerance > 0 ? opts.inner_tolerance : 0.1));
+  // true residual r = mask(b_eff - K x) into g_r (mixed path)
+  auto true_residual = [&](double* rr) -> int {
+    if (int rc = run_chunk(h, pa::kModeResidual, x, z, nullptr, h->m_R.p, b == nullptr)) return rc;
+    PA_HIP(pa::launch_neg_mask(h->m_R.p, b, h->bcmask.p, h->g_r.p, nn, PS(pa::kPDot), G, s));
+    pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot1);
+    PA_HIP(hipGetLastError());
+    if (int rc = read_scal(h)) return rc;
+    *rr = h->host_scal[pa::kDot1];
+    return PA_OK;
+  };
+  if (mixed) {
+    if (int rc = mixed_setup(h, z)) return rc;
+    if (!opts.use_x0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
+    const int nbc = (int)h->bc_dofs.size();
+    if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(x, h->bc_d.p, h->bc_v.p, nbc, b ? 0.0 : 1.0);
+    PA_HIP(hipGetLastError());
+    bb = h->f_free_norm2;
+    if (b) {  // ||mask(b)||^2
+      pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(h->m_R.p, ndof, 0.0);
+      PA_HIP(pa::launch_neg_mask(h->m_R.p, b, h->bcmask.p, h->g_r.p, nn, PS(pa::kPDot), G, s));
+      pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot1);
+      PA_HIP(hipGetLastError());
+      if (int rc = read_scal(h)) return rc;
+      bb = h->host_scal[pa::kDot1];
+    }
+    if (int rc = true_residual(&rr0)) return rc;
+  } else {
+    pa::k_diag_inv<<<grid_for(nn), 256, 0, s>>>(vals0, h->diag_slot.p, h->w_dinv.p, nn);
+    if (opts.use_x0) {
+      pa::k_spmv<<<pa::red_grid((size_t)nn, 16), 256, 0, s>>>(vals0, h->rowmap.p, h->colidx.p, x, b, h->g_r.p, nn, 1);
+    } else {
+      pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
+      PA_HIP(hipMemcpyAsync(h->g_r.p, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    PA_HIP(hipGetLastError());
+    if (int rc = mg_dot(h, b, b, ndof, pa::kDot0, &bb)) return rc;
+    if (int rc = mg_dot(h, h->g_r.p, h->g_r.p, ndof, pa::kDot0, &rr0)) return rc;
+  }
+  if (int rc = mg_setup(h, z, vals0, mixed)) return rc;
+  double rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
+  if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
+  bool conv = bb == 0.0 || rel <= opts.tolerance;
+  int it = 0, replacements = 0;
+  double rr_ref = rr0;
+  // z = V(r) and rz; kBetaInit also starts p = z, kBetaReplace recomputes beta for a
+  // replaced residual (the direction update p = z + beta p follows in the loop)
+  auto precondition = [&](int mode) -> int {
+    if (int rc = mg_vcycle(h, 0, h->g_r.p, h->g_z.p, vals0, mixed)) return rc;
+    pa::k_dot<<<G, 256, 0, s>>>(h->g_r.p, h->g_z.p, ndof, PS(pa::kPRZ0));
+    PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), mode == pa::kBetaStep ? PS(pa::kPRR) : nullptr, G, h->scal.p, mode,
+                                s));
+    if (mode == pa::kBetaInit)
+      PA_HIP(hipMemcpyAsync(h->g_p.p, h->g_z.p, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
+    return PA_OK;
+  };
+  if (!conv)
+    if (int rc = precondition(pa::kBetaInit)) return rc;
+  // a multigrid iteration reduces the residual several times over: check (and, mixed,
+  // replace the residual) at least every second iteration
+  const int check_every = std::min(opts.check_every, 2);
+  while (!conv && it < opts.max_iters) {
+    if (int rc = mg_apply(h, 0, h->g_p.p, h->g_q.p, vals0, mixed)) return rc;
+    pa::k_dot<<<G, 256, 0, s>>>(h->g_p.p, h->g_q.p, ndof, PS(pa::kPPQ));
+    PA_HIP(pa::launch_pcgm_alpha(PS(pa::kPPQ), G, h->scal.p, s));
+    PA_HIP(pa::launch_mg_update(x, h->g_r.p, h->g_p.p, h->g_q.p, h->scal.p, nn, PS(pa::kPRR), G, s));
+    if (int rc = precondition(pa::kBetaStep)) return rc;
+    ++it;
+    const bool last = it >= opts.max_iters;
+    bool restarted = false;
+    if (it % check_every == 0 || last) {
+      if (int rc = read_scal(h)) return rc;
+      const double rr_it = h->host_scal[pa::kRR];
+      if (mg_trace())
+        std::fprintf(stderr, "mg pcg it %d: iterated rel %.3e, rz %.3e, alpha %.3e, beta %.3e\n", it,
+                     std::sqrt(rr_it / bb), h->host_scal[pa::kRZ], h->host_scal[pa::kAlpha], h->host_scal[pa::kBeta]);
+      if (!std::isfinite(rr_it)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
+      if (!mixed) {
+        rel = std::sqrt(rr_it / bb);
+        conv = rel <= opts.tolerance;
+      } else if (rr_it <= delta * delta * rr_ref || rr_it <= opts.tolerance * opts.tolerance * bb || last) {
+        // replace the iterated residual by the true FP64 one
+        ++replacements;
+        if (int rc = true_residual(&rr_ref)) return rc;
+        rel = std::sqrt(rr_ref / bb);
+        if (mg_trace()) std::fprintf(stderr, "mg pcg it %d: true rel %.3e\n", it, rel);
+        if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
+        conv = rel <= opts.tolerance;
+        if (!conv && !last) {
+          // keep the direction unless the true residual is well above the iterated one
+          // (then the kept direction no longer matches r and PCG restarts from p = z)
+          restarted = rr_ref > 4.0 * rr_it;
+          if (int rc = precondition(restarted ? pa::kBetaInit : pa::kBetaReplace)) return rc;
+        }
+      }
+    }
+    if (!conv && !last && !restarted) PA_HIP(pa::launch_mg_pupdate(h->g_p.p, h->g_z.p, h->scal.p, ndof, s));
+  }
+  double true_rel = rel;
+  if (!mixed) {  // true FP64 residual of the assembled system
+    pa::k_spmv<<<pa::red_grid((size_t)nn, 16), 256, 0, s>>>(vals0, h->rowmap.p, h->colidx.p, x, b, h->g_q.p, nn, 1);
+    double rr = 0.0;
+    if (int rc = mg_dot(h, h->g_q.p, h->g_q.p, ndof, pa::kDot0, &rr)) return rc;
+    true_rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+  }
+  if (st) {
+    st->iterations = it;
+    st->outer_iterations = mixed ? replacements : (int)h->mg.size();
+    st->converged = conv ? 1 : 0;
+    st->rel_residual = true_rel;
+  }
+  if (!conv)
+    return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG stopped at relative residual " + std::to_string(rel));
   return PA_OK;
 }
 
@@ -754,7 +1266,11 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   if (!z || !u) return fail(PA_ERR_ARG, "pa_solution: null pointer");
   pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
-  if (opts.precision == 1) return solve_mixed(h, z, u, nullptr, opts, st);
+  if (opts.preconditioner != 0 && opts.preconditioner != 1)
+    return fail(PA_ERR_ARG, "preconditioner must be 0 (block Jacobi) or 1 (multigrid)");
+  const bool mg = opts.preconditioner == 1;
+  if (opts.precision == 1)
+    return mg ? solve_mg(h, z, u, nullptr, nullptr, opts, st, true) : solve_mixed(h, z, u, nullptr, opts, st);
   if (opts.precision != 0) return fail(PA_ERR_ARG, "precision must be 0 (FP64) or 1 (mixed)");
   if (!vals) return fail(PA_ERR_ARG, "pa_solution: FP64 solve needs the values buffer");
   if (int rc = ensure_solver_ws(h)) return rc;
@@ -768,6 +1284,7 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   if (int rc = run_chunk(h, pa::kModeJacobian, nullptr, z, vals, nullptr)) return rc;
   if (int rc = pa_apply_constraints(h, vals, rhs, 1.0)) return rc;
   opts.use_x0 = 0;
+  if (mg) return solve_mg(h, z, u, rhs, vals, opts, st, false);
   return pa_solve(h, vals, u, rhs, &opts, st);
 }
 
@@ -843,14 +1360,19 @@ int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int 
   double* dfdu = h->w_adj.p;
   if (int rc = pa_stress_pnorm(h, u, z, measure, pn, q, value, dfdu, dfdz)) return rc;
   opts.use_x0 = 0;
+  if (opts.preconditioner != 0 && opts.preconditioner != 1)
+    return fail(PA_ERR_ARG, "preconditioner must be 0 (block Jacobi) or 1 (multigrid)");
+  const bool mg = opts.preconditioner == 1;
   if (opts.precision == 1) {
-    if (int rc = solve_mixed(h, z, lambda, dfdu, opts, st)) return rc;
+    if (int rc = mg ? solve_mg(h, z, lambda, dfdu, nullptr, opts, st, true) : solve_mixed(h, z, lambda, dfdu, opts, st))
+      return rc;
   } else {
     const size_t nv = 9 * (size_t)h->graph.rowmap[h->nn];
     if (h->w_K.n != nv) PA_HIP(h->w_K.alloc(nv));
     if (int rc = run_chunk(h, pa::kModeJacobian, nullptr, z, h->w_K.p, nullptr)) return rc;
     if (int rc = pa_apply_constraints(h, h->w_K.p, dfdu, 0.0)) return rc;  // homogeneous on lambda
-    if (int rc = pa_solve(h, h->w_K.p, lambda, dfdu, &opts, st)) return rc;
+    if (int rc = mg ? solve_mg(h, z, lambda, dfdu, h->w_K.p, opts, st, false) : pa_solve(h, h->w_K.p, lambda, dfdu, &opts, st))
+      return rc;
   }
   if (h->w_adjz.n != (size_t)h->nn) PA_HIP(h->w_adjz.alloc(h->nn));
   if (int rc = pa_dRdz_T_apply(h, u, lambda, z, h->w_adjz.p)) return rc;

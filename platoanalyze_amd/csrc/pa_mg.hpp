@@ -1,0 +1,51 @@
+// Geometric multigrid on box node grids: transfer operators, density homogenisation,
+// Chebyshev block-Jacobi smoothing, the coarsest dense solve and the PCG vector steps
+// of the multigrid-preconditioned solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pa {
+
+struct BoxDims {
+  int nx, ny, nz;  // node counts per axis
+};
+
+// rc[c] = sum_{fine i within one fine spacing of coarse node c} w_i rf[i] with the
+// full-weighting weights (1, 1/2 per off-axis step), i.e. R = P^T of trilinear P;
+// constrained coarse dofs are set to 0.  3 components per node.
+hipError_t launch_mg_restrict(const double* rf, double* rc, const int32_t* cgrid, const int32_t* fnode_of_grid,
+                              BoxDims fd, const uint8_t* cbcmask, int ncn, hipStream_t s);
+// xf[i] += sum_parents w ec[parent] (trilinear), skipping constrained fine dofs.
+hipError_t launch_mg_prolong_add(const double* ec, double* xf, const int32_t* fgrid, const int32_t* cnode_of_grid,
+                                 BoxDims cd, const uint8_t* fbcmask, int nfn, hipStream_t s);
+// Normalised full weighting of a scalar nodal field (density homogenisation through the
+// SIMP stiffness weight): wc[c] = sum w_i wf[i] / sum w_i.
+hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgrid, const int32_t* fnode_of_grid,
+                               BoxDims fd, int ncn, hipStream_t s);
+// w = rmin + (1 - rmin) z^p and its inverse z = ((w - rmin) / (1 - rmin))^(1/p)
+hipError_t launch_simp_w(const double* z, double* w, int n, double rmin, double p, hipStream_t s);
+hipError_t launch_simp_z(const double* w, double* z, int n, double rmin, double p, hipStream_t s);
+
+// Chebyshev smoothing of K x = r with block-Jacobi D^-1, from x = 0:
+//   first: d = D^-1 r / theta; x = d
+//   next:  d = c1 d + c2 D^-1 (r - Kx); x += d
+hipError_t launch_cheb_first(double* x, double* d, const double* r, const double* dinv, double inv_theta, int nn,
+                             hipStream_t s);
+hipError_t launch_cheb_next(double* x, double* d, const double* r, const double* Kx, const double* dinv, double c1,
+                            double c2, int nn, hipStream_t s);
+// out = a - b ; y += x ; y = D^-1 x
+hipError_t launch_sub(double* out, const double* a, const double* b, size_t n, hipStream_t s);
+hipError_t launch_add(double* y, const double* x, size_t n, hipStream_t s);
+hipError_t launch_dinv_apply(double* y, const double* dinv, const double* x, int nn, hipStream_t s);
+// y = A x with A dense n x n (row major)
+hipError_t launch_dense_gemv(const double* A, const double* x, double* y, int n, hipStream_t s);
+
+// PCG steps of the multigrid-preconditioned solve (scalars in scal, see ScalarSlot):
+//   update: x += alpha p; r -= alpha q (alpha = scal[kAlpha]); partials of r.r
+//   pupdate: p = z + beta p (beta = scal[kBeta])
+hipError_t launch_mg_update(double* x, double* r, const double* p, const double* q, const double* scal, int nn,
+                            double* P_rr, int grid, hipStream_t s);
+hipError_t launch_mg_pupdate(double* p, const double* z, const double* scal, size_t n, hipStream_t s);
+
+}  // namespace pa

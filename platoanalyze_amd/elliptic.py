@@ -78,6 +78,7 @@ class ElasticModel:
     tractions: list = field(default_factory=list)
     graph: tuple | None = None  # optional (rowmap, colidx) host arrays
     device: str = "cuda"
+    box: tuple | None = None  # (nx, ny, nz) hexes of a structured box mesh: enables multigrid
 
     def __post_init__(self):
         h = C.c_void_p()
@@ -100,6 +101,8 @@ class ElasticModel:
             ci = np.ascontiguousarray(self.graph[1], np.int32)
             _capi.call("pa_set_graph", h, _hptr(rm), _hptr(ci))
         _capi.call("pa_finalize", h)
+        if self.box is not None:
+            _capi.call("pa_set_box", h, *(int(v) for v in self.box))
         nr, nb = C.c_int(), C.c_int()
         _capi.call("pa_graph_size", h, C.byref(nr), C.byref(nb))
         self.num_rows, self.nnz_blocks = nr.value, nb.value
@@ -204,15 +207,20 @@ class Problem:
     """Plato::Elliptic::Problem (self-adjoint compliance)."""
 
     def __init__(self, model: ElasticModel, tolerance: float = 1e-10, max_iters: int = 200000,
-                 check_every: int = 16, precision: str = "fp64", inner_tolerance: float = 0.1):
-        """precision "fp64": assembled block-CRS Jacobi-PCG in FP64.  "mixed": Jacobi-PCG
-        with FP64 vectors and an FP32 matrix-free operator, its residual replaced by the
-        true FP64 residual f - K u each time it has dropped by `inner_tolerance`."""
+                 check_every: int = 16, precision: str = "fp64", inner_tolerance: float = 0.1,
+                 preconditioner: str = "jacobi"):
+        """precision "fp64": assembled block-CRS PCG in FP64.  "mixed": PCG with FP64
+        vectors and an FP32 matrix-free operator, its residual replaced by the true FP64
+        residual f - K u each time it has dropped by `inner_tolerance`.
+        preconditioner "jacobi": 3x3 block Jacobi; "multigrid": geometric V-cycle on the
+        model's box (ElasticModel(box=(nx, ny, nz)))."""
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if preconditioner not in ("jacobi", "multigrid"):
+            raise ValueError("preconditioner must be 'jacobi' or 'multigrid'")
         self.m = model
         self.opts = _capi.SolverOpts(tolerance, max_iters, check_every, 0 if precision == "fp64" else 1, 0,
-                                     inner_tolerance)
+                                     inner_tolerance, 0 if preconditioner == "jacobi" else 1)
         self.stats = _capi.SolverStats()
         self._vals = None
         self.state = None
@@ -253,7 +261,7 @@ def solve(model: ElasticModel, values: torch.Tensor, b: torch.Tensor, tolerance:
           max_iters: int = 200000, x0: torch.Tensor | None = None):
     """AbstractSolver::solve(A, x, b) on a constrained matrix of the model's graph."""
     x = x0.clone() if x0 is not None else torch.empty_like(b)
-    opts = _capi.SolverOpts(tolerance, max_iters, 16, 0, 1 if x0 is not None else 0, 1e-4)
+    opts = _capi.SolverOpts(tolerance, max_iters, 16, 0, 1 if x0 is not None else 0, 1e-4, 0)
     st = _capi.SolverStats()
     model._stream()
     _capi.call("pa_solve", model.handle, _dptr(values), _dptr(x), _dptr(b), C.byref(opts), C.byref(st))

@@ -76,5 +76,6 @@ def model(w: Workload, device: str = "cuda", graph=None):
 
     bcs = [EssentialBC(w.fixed_nodes, d, 0.0) for d in range(3)]
     tr = [Traction(w.face_verts, w.face_elems, w.traction)]
-    return ElasticModel(w.mesh, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min, bcs, tr, graph=graph, device=device)
+    return ElasticModel(w.mesh, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min, bcs, tr, graph=graph, device=device,
+                        box=(w.cfg.nx, w.cfg.ny, w.cfg.nz))
 
