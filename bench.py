@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--profile-solve", type=int, default=0, metavar="K",
                     help="run only the solves (FP64/mixed x Jacobi/multigrid), each capped at K iterations "
                          "(for profilers)")
+    ap.add_argument("--solvers", default="fp64_jacobi,mixed_jacobi,fp64_multigrid,mixed_multigrid",
+                    help="comma-separated precision_preconditioner variants for --profile-solve")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the assembly/solve path runs on a single MI355X "
@@ -99,7 +101,8 @@ def main() -> int:
         m = wl.model(w, device=dev)
         z = torch.as_tensor(w.z, device=dev)
         out = {}
-        for prec, pre in (("fp64", "jacobi"), ("mixed", "jacobi"), ("fp64", "multigrid"), ("mixed", "multigrid")):
+        for name in args.solvers.split(","):
+            prec, pre = name.split("_")
             pb = Problem(m, tolerance=1e-8, check_every=32, precision=prec, preconditioner=pre,
                          max_iters=args.profile_solve)
             torch.cuda.synchronize()
