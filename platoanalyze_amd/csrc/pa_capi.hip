@@ -128,6 +128,15 @@ struct pa_problem {
   std::vector<int32_t> box_grid;  // (i, j, k) per node, host
   std::vector<std::unique_ptr<MgLevel>> mg;
   DevArray<double> mg_Ainv;
+  // Smoothing (measured at 128x64x64 and 256x128x128): on level 0 one damped block-Jacobi
+  // step before and after the coarse correction (Chebyshev degree 1: 52-54 iterations at
+  // 3 operator applications each beat degree 2's 48 at 5; unequal degrees break the
+  // symmetry PCG needs: 72-74), with the FP32 matrix-free operator also under the FP64
+  // solve (its CG operator stays the FP64 CRS matrix); degree 2 on coarser levels
+  // (degree 1 there: 74-76 iterations); interval [lambda / 60, 1.1 lambda].
+  bool mg_fp32_smooth = true;
+  int mg_pre0 = 1, mg_post0 = 1, mg_degc = 2;
+  double mg_ratio = 60.0;
   DevArray<double> g_x, g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
   double* host_scal = nullptr;
 
@@ -849,6 +858,17 @@ static int mg_level_init(pa_handle h, MgLevel& L, const std::vector<int32_t>& gr
 static int mg_build(pa_handle h) {
   if (!h->mg.empty()) return PA_OK;
   if (!h->has_box) return fail(PA_ERR_STATE, "multigrid needs the box grid (pa_set_box)");
+  // tuning knobs: PA_MG_FP32_SMOOTH=0|1, PA_MG_DEG0=<pre><post> (each 1 or 2) on level 0,
+  // PA_MG_DEGC=1|2 on the coarser levels, PA_MG_RATIO = lambda_max / lambda_min
+  if (const char* e = std::getenv("PA_MG_FP32_SMOOTH")) h->mg_fp32_smooth = std::atoi(e) != 0;
+  if (const char* e = std::getenv("PA_MG_DEG0")) {
+    if (std::strlen(e) == 2 && (e[0] == '1' || e[0] == '2') && (e[1] == '1' || e[1] == '2')) {
+      h->mg_pre0 = e[0] - '0';
+      h->mg_post0 = e[1] - '0';
+    }
+  }
+  if (const char* e = std::getenv("PA_MG_DEGC")) h->mg_degc = std::atoi(e) == 1 ? 1 : 2;
+  if (const char* e = std::getenv("PA_MG_RATIO")) h->mg_ratio = std::max(1.5, std::atof(e));
   auto L0 = std::make_unique<MgLevel>();
   L0->h = h;
   L0->dims = h->box;
@@ -950,9 +970,37 @@ static const double* mg_dinv(pa_handle h, size_t l, bool mixed) {
   return mixed ? h->m_dinv.p : h->w_dinv.p;
 }
 
+// Fused block-CRS smoothing step on level l (l > 0, or level 0 of the FP64 solve).
+static int mg_rows(pa_handle h, size_t l, int mode, const double* vals0, const double* x, const double* r,
+                   double* d, double* out, double* y, const double* base) {
+  MgLevel& L = *h->mg[l];
+  pa::MgRowArgs a{};
+  a.vals = l == 0 ? vals0 : L.vals.p;
+  a.rowmap = L.h->rowmap.p;
+  a.col = L.h->colidx.p;
+  a.x = x;
+  a.r = r;
+  a.dinv = mg_dinv(h, l, false);
+  a.d = d;
+  a.out = out;
+  a.y = y;
+  a.base = base;
+  a.c1 = L.c1;
+  a.c2 = L.c2;
+  a.inv_theta = L.inv_theta;
+  a.nrows = L.nn;
+  PA_HIP(pa::launch_mg_rows(mode, a, h->stream));
+  return PA_OK;
+}
+
 // x = V(r): symmetric V-cycle, degree-2 Chebyshev block-Jacobi pre- and post-smoothing,
 // full-weighting restriction, trilinear prolongation, dense solve on the last level.
-static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const double* vals0, bool mixed) {
+// `started`: the caller has already set x = d = D^-1 r / theta (fused into its
+// restriction).  Where the level has an assembled operator the smoothing steps are
+// fused with the operator product (6 launches per level); level 0 of the mixed solve
+// applies the FP32 matrix-free operator and runs the steps as separate kernels.
+static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const double* vals0, bool mixed,
+                     bool started = false) {
   hipStream_t s = h->stream;
   MgLevel& L = *h->mg[l];
   const size_t nd = 3 * (size_t)L.nn;
@@ -961,24 +1009,50 @@ static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const do
     return PA_OK;
   }
   MgLevel& C = *h->mg[l + 1];
+  const bool c_last = l + 2 == h->mg.size();
   const double* dinv = mg_dinv(h, l, mixed);
-  // pre-smoothing from x = 0
-  PA_HIP(pa::launch_cheb_first(x, L.d.p, r, dinv, L.inv_theta, L.nn, s));
-  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
-  PA_HIP(pa::launch_cheb_next(x, L.d.p, r, L.t.p, dinv, L.c1, L.c2, L.nn, s));
-  // coarse-grid correction
-  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
-  PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
-  PA_HIP(pa::launch_mg_restrict(L.rr.p, C.r.p, C.grid.p, L.node_of_grid.p, L.dims, C.h->bcmask.p, C.nn, s));
-  if (int rc = mg_vcycle(h, l + 1, C.r.p, C.x.p, vals0, mixed)) return rc;
-  PA_HIP(pa::launch_mg_prolong_add(C.x.p, x, L.grid.p, C.node_of_grid.p, C.dims, L.h->bcmask.p, L.nn, s));
+  const bool fused = !(l == 0 && mixed);
+  const int pre = l == 0 ? h->mg_pre0 : h->mg_degc, post = l == 0 ? h->mg_post0 : h->mg_degc;
+  if (!started) PA_HIP(pa::launch_cheb_first(x, L.d.p, r, dinv, L.inv_theta, L.nn, s));
+  double* x1 = x;  // iterate after pre-smoothing (and, later, the coarse correction)
+  if (fused) {
+    if (pre == 2) {
+      x1 = L.pw.p;
+      if (int rc = mg_rows(h, l, pa::kMgCheb, vals0, x, r, L.d.p, x1, nullptr, nullptr)) return rc;
+    }
+    if (int rc = mg_rows(h, l, pa::kMgRes, vals0, x1, r, nullptr, L.rr.p, nullptr, nullptr)) return rc;
+  } else {
+    if (pre == 2) {
+      if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+      PA_HIP(pa::launch_cheb_next(x, L.d.p, r, L.t.p, dinv, L.c1, L.c2, L.nn, s));
+    }
+    if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+    PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
+  }
+  // coarse-grid correction; the coarse pre-smoothing start is fused into the restriction
+  PA_HIP(pa::launch_mg_restrict(L.rr.p, C.r.p, C.grid.p, L.node_of_grid.p, L.dims, C.h->bcmask.p, C.nn, s,
+                                c_last ? nullptr : C.dinv.p, C.inv_theta, c_last ? nullptr : C.x.p,
+                                c_last ? nullptr : C.d.p));
+  if (int rc = mg_vcycle(h, l + 1, C.r.p, C.x.p, vals0, mixed, !c_last)) return rc;
+  PA_HIP(pa::launch_mg_prolong_add(C.x.p, x1, L.grid.p, C.node_of_grid.p, C.dims, L.h->bcmask.p, L.nn, s));
   // post-smoothing on the remaining residual
-  if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
-  PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
-  PA_HIP(pa::launch_cheb_first(L.y.p, L.d.p, L.rr.p, dinv, L.inv_theta, L.nn, s));
-  if (int rc = mg_apply(h, l, L.y.p, L.t.p, vals0, mixed)) return rc;
-  PA_HIP(pa::launch_cheb_next(L.y.p, L.d.p, L.rr.p, L.t.p, dinv, L.c1, L.c2, L.nn, s));
-  PA_HIP(pa::launch_add(x, L.y.p, nd, s));
+  if (fused) {
+    if (int rc = mg_rows(h, l, pa::kMgResCheb0, vals0, x1, r, L.d.p, L.rr.p, L.y.p, nullptr)) return rc;
+    if (post == 2) {
+      if (int rc = mg_rows(h, l, pa::kMgChebAdd, vals0, L.y.p, L.rr.p, L.d.p, x, nullptr, x1)) return rc;
+    } else {
+      PA_HIP(pa::launch_add2(x, x1, L.y.p, nd, s));
+    }
+  } else {
+    if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
+    PA_HIP(pa::launch_sub(L.rr.p, r, L.t.p, nd, s));
+    PA_HIP(pa::launch_cheb_first(L.y.p, L.d.p, L.rr.p, dinv, L.inv_theta, L.nn, s));
+    if (post == 2) {
+      if (int rc = mg_apply(h, l, L.y.p, L.t.p, vals0, mixed)) return rc;
+      PA_HIP(pa::launch_cheb_next(L.y.p, L.d.p, L.rr.p, L.t.p, dinv, L.c1, L.c2, L.nn, s));
+    }
+    PA_HIP(pa::launch_add(x, L.y.p, nd, s));
+  }
   return PA_OK;
 }
 
@@ -1069,18 +1143,18 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
     std::vector<double> v0(nd);
     for (size_t i = 0; i < nd; ++i) v0[i] = (double)((i * 2654435761ull >> 7) % 2001) / 1000.0 - 1.0;
     PA_HIP(hipMemcpyAsync(L.pw.p, v0.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
+    // v <- D^-1 K v / |D^-1 K v| on the device; the host reads |D^-1 K v| once
+    if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, nullptr)) return rc;
     double nrm = 0.0;
-    if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, &nrm)) return rc;
-    double lam = 0.0;
     for (int it = 0; it < 12; ++it) {
-      pa::k_scale<<<grid_for(nd), 256, 0, s>>>(L.pw.p, nd, 1.0 / std::sqrt(nrm));
+      PA_HIP(pa::launch_scale_rsqrt(L.pw.p, nd, h->scal.p, pa::kDot0, s));
       if (int rc = mg_apply(h, l, L.pw.p, L.t.p, vals0, mixed)) return rc;
       PA_HIP(pa::launch_dinv_apply(L.pw.p, dinv, L.t.p, L.nn, s));
-      if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, &nrm)) return rc;
-      lam = std::sqrt(nrm);
+      if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, it == 11 ? &nrm : nullptr)) return rc;
     }
+    const double lam = std::sqrt(nrm);
     if (!(lam > 0.0) || !std::isfinite(lam)) return fail(PA_ERR_NOT_CONVERGED, "multigrid: bad spectral estimate");
-    const double lmax = 1.1 * lam, lmin = lmax / 30.0;
+    const double lmax = 1.1 * lam, lmin = lmax / h->mg_ratio;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
     L.lam = lam;
@@ -1171,7 +1245,12 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
     if (int rc = mg_dot(h, b, b, ndof, pa::kDot0, &bb)) return rc;
     if (int rc = mg_dot(h, h->g_r.p, h->g_r.p, ndof, pa::kDot0, &rr0)) return rc;
   }
-  if (int rc = mg_setup(h, z, vals0, mixed)) return rc;
+  // level-0 smoother operator: FP32 matrix-free under the mixed solve (and, if chosen,
+  // under the FP64 solve, whose CG operator stays the FP64 CRS matrix)
+  const bool mf0 = mixed || h->mg_fp32_smooth;
+  if (mf0 && !mixed)
+    if (int rc = mixed_setup(h, z)) return rc;
+  if (int rc = mg_setup(h, z, vals0, mf0)) return rc;
   double rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
   if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
   bool conv = bb == 0.0 || rel <= opts.tolerance;
@@ -1180,7 +1259,7 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
   // z = V(r) and rz; kBetaInit also starts p = z, kBetaReplace recomputes beta for a
   // replaced residual (the direction update p = z + beta p follows in the loop)
   auto precondition = [&](int mode) -> int {
-    if (int rc = mg_vcycle(h, 0, h->g_r.p, h->g_z.p, vals0, mixed)) return rc;
+    if (int rc = mg_vcycle(h, 0, h->g_r.p, h->g_z.p, vals0, mf0)) return rc;
     pa::k_dot<<<G, 256, 0, s>>>(h->g_r.p, h->g_z.p, ndof, PS(pa::kPRZ0));
     PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), mode == pa::kBetaStep ? PS(pa::kPRR) : nullptr, G, h->scal.p, mode,
                                 s));

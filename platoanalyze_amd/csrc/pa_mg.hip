@@ -8,6 +8,7 @@
 
 #include "pa_reduce.hpp"
 #include "pa_solver.hpp"
+#include "pa_spmv.hpp"
 
 namespace pa {
 
@@ -20,10 +21,18 @@ int blocks_of(size_t n, int cap = 4096) {
   return (int)(g < 1 ? 1 : (g > (size_t)cap ? cap : g));
 }
 
+__device__ __forceinline__ void dinv3(const double* __restrict__ D, const double r[3], double z[3]) {
+  z[0] = D[0] * r[0] + D[1] * r[1] + D[2] * r[2];
+  z[1] = D[3] * r[0] + D[4] * r[1] + D[5] * r[2];
+  z[2] = D[6] * r[0] + D[7] * r[1] + D[8] * r[2];
+}
+
 __global__ __launch_bounds__(kT) void k_mg_restrict(const double* __restrict__ rf, double* __restrict__ rc,
                                                     const int32_t* __restrict__ cgrid,
                                                     const int32_t* __restrict__ fnode, BoxDims fd,
-                                                    const uint8_t* __restrict__ cbc, int ncn) {
+                                                    const uint8_t* __restrict__ cbc, int ncn,
+                                                    const double* __restrict__ cdinv, double inv_theta,
+                                                    double* __restrict__ cx, double* __restrict__ cd) {
   for (int c = blockIdx.x * kT + threadIdx.x; c < ncn; c += gridDim.x * kT) {
     const int I = cgrid[3 * c], J = cgrid[3 * c + 1], K = cgrid[3 * c + 2];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
@@ -45,9 +54,20 @@ __global__ __launch_bounds__(kT) void k_mg_restrict(const double* __restrict__ r
       }
     }
     const uint32_t bm = cbc[c];
-    rc[3 * (size_t)c] = (bm & 1u) ? 0.0 : s0;
-    rc[3 * (size_t)c + 1] = (bm & 2u) ? 0.0 : s1;
-    rc[3 * (size_t)c + 2] = (bm & 4u) ? 0.0 : s2;
+    const double v[3] = {(bm & 1u) ? 0.0 : s0, (bm & 2u) ? 0.0 : s1, (bm & 4u) ? 0.0 : s2};
+    const size_t o = 3 * (size_t)c;
+    rc[o] = v[0];
+    rc[o + 1] = v[1];
+    rc[o + 2] = v[2];
+    if (cx) {
+      double z[3];
+      dinv3(cdinv + 9 * (size_t)c, v, z);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cx[o + i] = inv_theta * z[i];
+        cd[o + i] = inv_theta * z[i];
+      }
+    }
   }
 }
 
@@ -114,12 +134,6 @@ __global__ __launch_bounds__(kT) void k_simp_z(const double* w, double* z, int n
   }
 }
 
-__device__ __forceinline__ void dinv3(const double* __restrict__ D, const double r[3], double z[3]) {
-  z[0] = D[0] * r[0] + D[1] * r[1] + D[2] * r[2];
-  z[1] = D[3] * r[0] + D[4] * r[1] + D[5] * r[2];
-  z[2] = D[6] * r[0] + D[7] * r[1] + D[8] * r[2];
-}
-
 __global__ __launch_bounds__(kT) void k_cheb_first(double* __restrict__ x, double* __restrict__ d,
                                                    const double* __restrict__ r, const double* __restrict__ dinv,
                                                    double inv_theta, int nn) {
@@ -160,6 +174,10 @@ __global__ __launch_bounds__(kT) void k_sub(double* out, const double* a, const 
 
 __global__ __launch_bounds__(kT) void k_add(double* y, const double* x, size_t n) {
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] += x[i];
+}
+
+__global__ __launch_bounds__(kT) void k_add2(double* out, const double* a, const double* b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) out[i] = a[i] + b[i];
 }
 
 __global__ __launch_bounds__(kT) void k_dinv_apply(double* __restrict__ y, const double* __restrict__ dinv,
@@ -216,11 +234,80 @@ __global__ __launch_bounds__(kT) void k_mg_pupdate(double* __restrict__ p, const
     p[i] = fma(beta, p[i], z[i]);
 }
 
+template <int MODE>
+__device__ __forceinline__ void mg_row_epilogue(const MgRowArgs& A, int row, const double acc[3]) {
+  {
+    const double* __restrict__ x = A.x;
+    const size_t o = 3 * (size_t)row;
+    const double t[3] = {A.r[o] - acc[0], A.r[o + 1] - acc[1], A.r[o + 2] - acc[2]};
+    if constexpr (MODE == kMgRes) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) A.out[o + i] = t[i];
+    } else if constexpr (MODE == kMgResCheb0) {
+      double z[3];
+      dinv3(A.dinv + 9 * (size_t)row, t, z);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        A.out[o + i] = t[i];
+        A.y[o + i] = A.inv_theta * z[i];
+        A.d[o + i] = A.inv_theta * z[i];
+      }
+    } else {
+      double z[3];
+      dinv3(A.dinv + 9 * (size_t)row, t, z);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double dn = A.c1 * A.d[o + i] + A.c2 * z[i];
+        A.d[o + i] = dn;
+        if constexpr (MODE == kMgCheb) A.out[o + i] = x[o + i] + dn;
+        else A.out[o + i] = A.base[o + i] + x[o + i] + dn;
+      }
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kT) void k_mg_rows(MgRowArgs A) {
+  constexpr int kRows = kT / kLanesPerRow;
+  const int lane = threadIdx.x % kLanesPerRow;
+  const XcdSpan sp = xcd_span(A.nrows);
+  const double* __restrict__ x = A.x;
+  auto X = [&](int c, int j) { return x[3 * (size_t)c + j]; };
+  for (int row = sp.lo + sp.j * kRows + threadIdx.x / kLanesPerRow; row < sp.hi; row += sp.bx * kRows) {
+    double acc[3];
+    spmv_row(A.vals, A.rowmap, A.col, row, lane, X, acc);
+    if (lane == 0) mg_row_epilogue<MODE>(A, row, acc);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_scale_rsqrt(double* __restrict__ y, size_t n, const double* __restrict__ scal,
+                                                    int slot) {
+  const double f = 1.0 / sqrt(scal[slot]);
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] *= f;
+}
+
 }  // namespace
 
 hipError_t launch_mg_restrict(const double* rf, double* rc, const int32_t* cgrid, const int32_t* fnode_of_grid,
-                              BoxDims fd, const uint8_t* cbcmask, int ncn, hipStream_t s) {
-  k_mg_restrict<<<blocks_of(ncn), kT, 0, s>>>(rf, rc, cgrid, fnode_of_grid, fd, cbcmask, ncn);
+                              BoxDims fd, const uint8_t* cbcmask, int ncn, hipStream_t s, const double* cdinv,
+                              double inv_theta, double* cx, double* cd) {
+  k_mg_restrict<<<blocks_of(ncn), kT, 0, s>>>(rf, rc, cgrid, fnode_of_grid, fd, cbcmask, ncn, cdinv, inv_theta, cx,
+                                              cd);
+  return hipGetLastError();
+}
+hipError_t launch_mg_rows(int mode, const MgRowArgs& a, hipStream_t s) {
+  const int grid = red_grid((size_t)a.nrows, kT / kLanesPerRow);
+  switch (mode) {
+    case kMgRes: k_mg_rows<kMgRes><<<grid, kT, 0, s>>>(a); break;
+    case kMgResCheb0: k_mg_rows<kMgResCheb0><<<grid, kT, 0, s>>>(a); break;
+    case kMgCheb: k_mg_rows<kMgCheb><<<grid, kT, 0, s>>>(a); break;
+    case kMgChebAdd: k_mg_rows<kMgChebAdd><<<grid, kT, 0, s>>>(a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+hipError_t launch_scale_rsqrt(double* y, size_t n, const double* scal, int slot, hipStream_t s) {
+  k_scale_rsqrt<<<blocks_of(n), kT, 0, s>>>(y, n, scal, slot);
   return hipGetLastError();
 }
 hipError_t launch_mg_prolong_add(const double* ec, double* xf, const int32_t* fgrid, const int32_t* cnode_of_grid,
@@ -257,6 +344,10 @@ hipError_t launch_sub(double* out, const double* a, const double* b, size_t n, h
 }
 hipError_t launch_add(double* y, const double* x, size_t n, hipStream_t s) {
   k_add<<<blocks_of(n), kT, 0, s>>>(y, x, n);
+  return hipGetLastError();
+}
+hipError_t launch_add2(double* out, const double* a, const double* b, size_t n, hipStream_t s) {
+  k_add2<<<blocks_of(n), kT, 0, s>>>(out, a, b, n);
   return hipGetLastError();
 }
 hipError_t launch_dinv_apply(double* y, const double* dinv, const double* x, int nn, hipStream_t s) {

@@ -14,8 +14,12 @@ struct BoxDims {
 // rc[c] = sum_{fine i within one fine spacing of coarse node c} w_i rf[i] with the
 // full-weighting weights (1, 1/2 per off-axis step), i.e. R = P^T of trilinear P;
 // constrained coarse dofs are set to 0.  3 components per node.
+// When cx is given, the coarse level's pre-smoothing starts in the same kernel:
+// cx = cd = D_c^-1 rc * inv_theta.
 hipError_t launch_mg_restrict(const double* rf, double* rc, const int32_t* cgrid, const int32_t* fnode_of_grid,
-                              BoxDims fd, const uint8_t* cbcmask, int ncn, hipStream_t s);
+                              BoxDims fd, const uint8_t* cbcmask, int ncn, hipStream_t s,
+                              const double* cdinv = nullptr, double inv_theta = 0.0, double* cx = nullptr,
+                              double* cd = nullptr);
 // xf[i] += sum_parents w ec[parent] (trilinear), skipping constrained fine dofs.
 hipError_t launch_mg_prolong_add(const double* ec, double* xf, const int32_t* fgrid, const int32_t* cnode_of_grid,
                                  BoxDims cd, const uint8_t* fbcmask, int nfn, hipStream_t s);
@@ -37,9 +41,34 @@ hipError_t launch_cheb_next(double* x, double* d, const double* r, const double*
 // out = a - b ; y += x ; y = D^-1 x
 hipError_t launch_sub(double* out, const double* a, const double* b, size_t n, hipStream_t s);
 hipError_t launch_add(double* y, const double* x, size_t n, hipStream_t s);
+hipError_t launch_add2(double* out, const double* a, const double* b, size_t n, hipStream_t s);  // out = a + b
 hipError_t launch_dinv_apply(double* y, const double* dinv, const double* x, int nn, hipStream_t s);
 // y = A x with A dense n x n (row major)
 hipError_t launch_dense_gemv(const double* A, const double* x, double* y, int n, hipStream_t s);
+
+// Smoothing steps fused with the block-CRS product K x (16 lanes per block row):
+//   kMgRes:      out = r - K x
+//   kMgResCheb0: out = r - K x;  y = d = D^-1 out * inv_theta   (post-smoothing start)
+//   kMgCheb:     d = c1 d + c2 D^-1 (r - K x);  out = x + d     (out must not alias x)
+//   kMgChebAdd:  d = c1 d + c2 D^-1 (r - K x);  out = base + x + d
+enum MgRowMode { kMgRes = 0, kMgResCheb0 = 1, kMgCheb = 2, kMgChebAdd = 3 };
+struct MgRowArgs {
+  const double* vals;
+  const int32_t* rowmap;
+  const int32_t* col;
+  const double* x;
+  const double* r;
+  const double* dinv;
+  double* d;
+  double* out;
+  double* y;
+  const double* base;
+  double c1, c2, inv_theta;
+  int nrows;
+};
+hipError_t launch_mg_rows(int mode, const MgRowArgs& a, hipStream_t s);
+// y *= 1 / sqrt(scal[slot])  (power-iteration normalisation without a host round trip)
+hipError_t launch_scale_rsqrt(double* y, size_t n, const double* scal, int slot, hipStream_t s);
 
 // PCG steps of the multigrid-preconditioned solve (scalars in scal, see ScalarSlot):
 //   update: x += alpha p; r -= alpha q (alpha = scal[kAlpha]); partials of r.r
