@@ -218,9 +218,10 @@ int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int 
  * No counterpart in the reference (whose Problem::solution always starts from u = 0):
  * a 4-level 3-D convolutional U-Net on the node grid of a box mesh, trained by the
  * library's own BF16-MFMA forward/backward kernels, whose output warm-starts the solve
- * (pa_solution with opts.use_x0 = 1).  Tensors are device FP32, channels last,
- * [batch][nx][ny][nz][8]: input channels density, load (3), coordinates (3), SIMP weight;
- * output channels u (3) then 5 padding channels (trained towards 0).  Node grid
+ * (pa_solution with opts.use_x0 = 1).  Tensors are device FP32, channels last:
+ * input [batch][nx][ny][nz][8] — density, load (3), coordinates (3), SIMP weight;
+ * output [batch][nx][ny][nz][16] — u (3), or u for three unit load cases (9), then
+ * padding channels (trained towards 0).  Node grid
  * dimensions must be 1 mod 4.  Parameters: FP32 vector of pa_unet_num_params values
  * (per layer: W[27][cin][cout] then bias[cout]). */
 typedef struct pa_unet_s* pa_unet;

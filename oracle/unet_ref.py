@@ -13,7 +13,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN_CH, OUT_CH = 8, 8
+IN_CH, OUT_CH = 8, 16
 
 
 def bf16(x):

@@ -313,7 +313,7 @@ using pa::unet::bf16_t;
 using pa::unet::Grid;
 
 constexpr int kInCh = 8;   // density, load (3), coordinates (3), ones
-constexpr int kOutCh = 8;  // displacement (3) + padding to the MFMA k granularity
+constexpr int kOutCh = 16;  // displacement channels (3, or 9 for three load cases) + padding
 
 // f32 -> bf16 conversion of the network input
 __global__ void k_to_bf16(const float* x, bf16_t* y, size_t n) {

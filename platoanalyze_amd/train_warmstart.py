@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 
-def evaluate(net, c0, held, tol=1e-8, log=print):
+def evaluate(net, c0, held, tol=1e-8, log=print, mode=None):
     from . import workloads as wl
     from .elliptic import Problem
     from .warmstart import WarmStart
@@ -27,7 +27,7 @@ def evaluate(net, c0, held, tol=1e-8, log=print):
     rows = []
     for w, z, f in held:
         m = wl.model(w)
-        ws = WarmStart(net, w, m, c0)
+        ws = WarmStart(net, w, m, c0, mode=mode)
         pb = Problem(m, tolerance=tol, check_every=16, precision="mixed")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -57,7 +57,7 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--channels", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--mode", default="load", choices=["load", "shape"])
+    ap.add_argument("--mode", default="load", choices=["load", "shape", "basis"])
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -72,8 +72,8 @@ def main(argv=None) -> int:
                             seed=a.seed, mode=a.mode, log=log)
     log(f"training done in {time.perf_counter() - t0:.1f} s")
     idx = np.linspace(0, len(held) - 1, min(5, len(held))).astype(int)
-    rows = evaluate(net, c0, [held[i] for i in idx], log=log)
-    meta = {"config": base.cfg.name, "traj": a.traj, "iters": a.iters, "epochs": a.epochs,
+    rows = evaluate(net, c0, [held[i] for i in idx], log=log, mode=a.mode)
+    meta = {"config": base.cfg.name, "mode": a.mode, "traj": a.traj, "iters": a.iters, "epochs": a.epochs,
             "loss_history": np.array(hist), "eval_iters_cold": np.array([r["iters_cold"] for r in rows]),
             "eval_iters_warm": np.array([r["iters_warm"] for r in rows])}
     save(net, c0, a.out, meta)

@@ -1,7 +1,8 @@
 """Python handle of the warm-start U-Net (pa_unet_* in include/plato_mi355x.h).
 
 All compute runs in the library's HIP kernels (BF16 MFMA forward and backward,
-FP32 Adam); tensors are torch CUDA float32, channels last: [batch, nx, ny, nz, 8].
+FP32 Adam); tensors are torch CUDA float32, channels last: input [batch, nx, ny, nz,
+IN_CH], output [batch, nx, ny, nz, OUT_CH].
 """
 from __future__ import annotations
 
@@ -12,7 +13,7 @@ import torch
 from . import _capi
 
 IN_CH = 8
-OUT_CH = 8
+OUT_CH = 16
 
 
 def _p(t: torch.Tensor):
@@ -36,8 +37,11 @@ class UNet:
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
-            _capi.lib().pa_unet_destroy(h)
             self.h = None
+            try:
+                _capi.lib().pa_unet_destroy(h)
+            except Exception:  # interpreter shutdown: the library may already be gone
+                pass
 
     def _stream(self):
         _capi.ucall("pa_unet_set_stream", self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream))

@@ -63,12 +63,13 @@ class Features:
         X = torch.as_tensor(w.mesh.coords, dtype=torch.float32, device=device)
         self.xyz = X / torch.as_tensor([w.cfg.lx, w.cfg.lx, w.cfg.lx], dtype=torch.float32, device=device)
 
-    def inputs(self, z: torch.Tensor, f: torch.Tensor) -> torch.Tensor:
+    def inputs(self, z: torch.Tensor, f: torch.Tensor | None) -> torch.Tensor:
+        """Network input; f = None (basis mode) leaves the load channels at zero."""
         x = torch.zeros((self.V, IN_CH), dtype=torch.float32, device=z.device)
-        fn = f.reshape(-1, 3).to(torch.float32)
-        fn = fn / fn.abs().max().clamp_min(1e-30)
         x[self.vox, 0] = z.to(torch.float32)
-        x[self.vox, 1:4] = fn
+        if f is not None:
+            fn = f.reshape(-1, 3).to(torch.float32)
+            x[self.vox, 1:4] = fn / fn.abs().max().clamp_min(1e-30)
         x[self.vox, 4:7] = self.xyz
         x[self.vox, 7] = (self.rmin + (1.0 - self.rmin) * z.pow(self.p)).to(torch.float32)  # SIMP stiffness
         return x.reshape(*self.dims, IN_CH)
@@ -87,6 +88,21 @@ class Features:
         """Network output -> displacement in node order (3N, float64)."""
         u = y.reshape(self.V, OUT_CH)[self.vox, 0:3].reshape(-1).to(torch.float64)
         return u * (self.load_scale(f) / c0)
+
+
+    def target_basis(self, us, s: float) -> torch.Tensor:
+        """Basis mode: the responses to unit tractions along x, y, z (3N each), times s,
+        in output channels 0-2, 3-5, 6-8."""
+        y = torch.zeros((self.V, OUT_CH), dtype=torch.float32, device=us[0].device)
+        for d, u in enumerate(us):
+            y[self.vox, 3 * d:3 * d + 3] = (u * s).reshape(-1, 3).to(torch.float32)
+        return y.reshape(*self.dims, OUT_CH)
+
+    def displacement_basis(self, y: torch.Tensor, t, s: float) -> torch.Tensor:
+        """Basis mode: u = sum_d t_d u_d from the network output (3N, float64), / s."""
+        Y = y.reshape(self.V, OUT_CH)[self.vox].to(torch.float64)
+        u = sum(float(t[d]) * Y[:, 3 * d:3 * d + 3] for d in range(3))
+        return u.reshape(-1) / s
 
 
 def galerkin_scale(model, z: torch.Tensor, u_hat: torch.Tensor, f: torch.Tensor, bc_dofs: torch.Tensor) -> float:
@@ -111,13 +127,17 @@ class WarmStart:
     """Trained network + the per-mesh feature map; `initial_guess` gives u0.
     mode "load": the network predicts the load-scaled displacement (c0 > 0);
     mode "shape": it predicts the unit-RMS shape and the amplitude is the Galerkin
-    factor (c0 <= 0)."""
+    factor (c0 <= 0);
+    mode "basis": it predicts, from the design alone, the responses to unit tractions
+    along x, y and z on the loaded face, combined with the problem's traction vector
+    outside the network (u is linear in the load, exactly)."""
 
-    def __init__(self, net: UNet, w: wl.Workload, model, c0: float, smooth_steps: int = 8):
+    def __init__(self, net: UNet, w: wl.Workload, model, c0: float, smooth_steps: int = 8,
+                 mode: str | None = None):
         self.net = net
         self.smooth_steps = int(smooth_steps)
         self.c0 = float(c0)
-        self.mode = "load" if self.c0 > 0 else "shape"
+        self.mode = mode or ("load" if self.c0 > 0 else "shape")
         self.w = w
         self.model = model
         self.feat = Features(w)
@@ -128,6 +148,13 @@ class WarmStart:
     def initial_guess(self, z: torch.Tensor, f: torch.Tensor | None = None) -> torch.Tensor:
         if f is None:
             f = load_vector(self.model, z)
+        if self.mode == "basis":
+            t = np.asarray(self.w.traction, dtype=np.float64)
+            fs = self.feat.load_scale(f) / max(float(np.abs(t).max()), 1e-300)  # unit-traction load scale
+            y = self.net.forward(self.feat.inputs(z, None).unsqueeze(0).contiguous())
+            u0 = self.feat.displacement_basis(y, t, self.c0 / fs)
+            u0[self.bc] = 0.0
+            return u0
         x = self.feat.inputs(z, f).unsqueeze(0).contiguous()
         y = self.net.forward(x)
         if self.mode == "load":
@@ -210,10 +237,79 @@ def trajectories(base: wl.Workload, n_traj: int, n_iter: int, seed: int = 7, tol
             f"({time.perf_counter() - t0:.1f} s)")
 
 
+UNIT_TRACTIONS = ((1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0))
+
+
+def basis_trajectories(base: wl.Workload, n_traj: int, n_iter: int, seed: int = 7, tol: float = 1e-8,
+                       log=print):
+    """SIMP OC runs whose designs are solved for unit tractions along x, y and z (three
+    multigrid solves); the trajectory's own load case is their combination.  Yields
+    (trajectory, iteration, workload with the trajectory's traction, z, [u_x, u_y, u_z])."""
+    rng = np.random.default_rng(seed)
+    wn = nodal_volumes(base)
+    models = [wl.model(dataclasses.replace(base, traction=t)) for t in UNIT_TRACTIONS]
+    pbs = [Problem(m, tolerance=tol, precision="mixed", preconditioner="multigrid") for m in models]
+    for t in range(n_traj):
+        vf = float(rng.uniform(0.3, 0.5))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        w = dataclasses.replace(base, traction=tuple(float(v) for v in d))
+        z = np.full(base.mesh.num_nodes, vf)
+        t0 = time.perf_counter()
+        for k in range(n_iter):
+            zt = torch.as_tensor(z, device="cuda")
+            us = [pb.solution(zt).clone() for pb in pbs]
+            u = sum(float(d[i]) * us[i] for i in range(3))
+            g, Cv = pbs[0].criterionGradient(zt, u)  # dC/dz depends on u and z only
+            yield t, k, w, zt, us
+            z = oc_update(z, g.cpu().numpy(), wn, vf, base.cfg.rho_min)
+        log(f"trajectory {t}: vf {vf:.3f} dir {np.round(d, 3)} compliance {Cv:.4g} "
+            f"({time.perf_counter() - t0:.1f} s)")
+
+
+def train_basis(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch: int = 4, lr: float = 1e-3,
+                base_channels: int = 16, seed: int = 0, holdout: int = 1, log=print):
+    """Basis mode (see WarmStart): returns (net, c0, history, held-out [(w, z, f)])."""
+    feat = Features(base)
+    m0 = wl.model(dataclasses.replace(base, traction=UNIT_TRACTIONS[0]))
+    fs = feat.load_scale(load_vector(m0, torch.full((base.mesh.num_nodes,), 0.5, dtype=torch.float64,
+                                                    device="cuda")))
+    X, U, held = [], [], []
+    for t, k, w, z, us in basis_trajectories(base, n_traj + holdout, n_iter, seed=seed + 11, log=log):
+        if t < n_traj:
+            X.append(feat.inputs(z, None))
+            U.append([(u / fs).to(torch.float32) for u in us])
+        else:
+            mw = wl.model(w)
+            held.append((w, z.clone(), load_vector(mw, z)))
+    c0 = float(1.0 / torch.stack([torch.stack([v.pow(2).mean() for v in us]).mean() for us in U]).mean().sqrt())
+    X = torch.stack(X)
+    Y = torch.stack([feat.target_basis([v.to(torch.float64) for v in us], c0) for us in U])
+    del U
+    net = UNet(feat.dims, base_channels, seed=seed)
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = X.shape[0]
+    for ep in range(epochs):
+        lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs, 1)))
+        perm = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        t0 = time.perf_counter()
+        for i in range(0, n - batch + 1, batch):
+            idx = torch.as_tensor(perm[i:i + batch], device="cuda")
+            tot += net.train_step(X[idx].contiguous(), Y[idx].contiguous(), lr_ep)
+            cnt += 1
+        hist.append(tot / max(cnt, 1))
+        log(f"epoch {ep}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s, {cnt} steps)")
+    return net, c0, hist, held
+
+
 def train(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch: int = 4, lr: float = 1e-3,
           base_channels: int = 16, seed: int = 0, holdout: int = 1, mode: str = "load", log=print):
     """Generate data along trajectories, train, and return (net, c0, history, held-out
     set); c0 <= 0 marks the "shape" mode (see WarmStart)."""
+    if mode == "basis":
+        return train_basis(base, n_traj, n_iter, epochs, batch, lr, base_channels, seed, holdout, log)
     feat = Features(base)
     X, U, held = [], [], []
     for t, k, w, m, z, f, u in trajectories(base, n_traj + holdout, n_iter, seed=seed + 11, log=log):

@@ -18,8 +18,8 @@ def _setup(seed=0, batch=2):
     net = UNet(GRID, BASE, seed=seed)
     rng = np.random.default_rng(seed + 10)
     V = int(np.prod(GRID))
-    x = rng.normal(size=(batch, V, 8)).astype(np.float32)
-    y = rng.normal(size=(batch, V, 8)).astype(np.float32) * 0.1
+    x = rng.normal(size=(batch, V, U.IN_CH)).astype(np.float32)
+    y = rng.normal(size=(batch, V, U.OUT_CH)).astype(np.float32) * 0.1
     return net, x, y
 
 
@@ -33,8 +33,8 @@ def test_gpu_unet_forward_matches_oracle():
     p = net.params().cpu().numpy().astype(np.float64)
     ref = U.UNetRef(GRID, BASE)
     out_ref, _ = ref.forward(x.astype(np.float64), p)
-    xt = torch.from_numpy(x).reshape(2, *GRID, 8).cuda()
-    out = net.forward(xt).reshape(2, -1, 8).cpu().numpy()
+    xt = torch.from_numpy(x).reshape(2, *GRID, U.IN_CH).cuda()
+    out = net.forward(xt).reshape(2, -1, U.OUT_CH).cpu().numpy()
     err = _rel(out, out_ref)
     print(f"forward rel L2 error {err:.2e}", flush=True)
     assert err < 1e-2
@@ -46,8 +46,8 @@ def test_gpu_unet_gradient_matches_oracle():
     p = net.params().cpu().numpy().astype(np.float64)
     ref = U.UNetRef(GRID, BASE)
     loss_ref, g_ref, _ = ref.loss_grad(x.astype(np.float64), y.astype(np.float64), p)
-    xt = torch.from_numpy(x).reshape(2, *GRID, 8).cuda()
-    yt = torch.from_numpy(y).reshape(2, *GRID, 8).cuda()
+    xt = torch.from_numpy(x).reshape(2, *GRID, U.IN_CH).cuda()
+    yt = torch.from_numpy(y).reshape(2, *GRID, U.OUT_CH).cuda()
     loss, g = net.gradient(xt, yt)
     g = g.cpu().numpy()
     err = _rel(g, g_ref)
@@ -65,8 +65,8 @@ def test_gpu_unet_gradient_matches_oracle():
 def test_gpu_unet_training_reduces_loss():
     print("case: unet overfit one batch", flush=True)
     net, x, y = _setup(seed=2)
-    xt = torch.from_numpy(x).reshape(2, *GRID, 8).cuda()
-    yt = torch.from_numpy(y).reshape(2, *GRID, 8).cuda()
+    xt = torch.from_numpy(x).reshape(2, *GRID, U.IN_CH).cuda()
+    yt = torch.from_numpy(y).reshape(2, *GRID, U.OUT_CH).cuda()
     l0 = net.train_step(xt, yt, 1e-3)
     for _ in range(200):
         l1 = net.train_step(xt, yt, 1e-3)

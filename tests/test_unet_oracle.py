@@ -56,8 +56,8 @@ def test_backward_matches_central_differences():
     for (W, b), l in zip(r.split(np.arange(n)), r.L):  # He-normal weights, small biases
         p[W.ravel()] = rng.normal(size=W.size) * np.sqrt(2.0 / (27 * l[2]))
         p[b] = rng.normal(size=b.size) * 0.05
-    x = rng.normal(size=(2, 9 * 5 * 5, 8))
-    y = rng.normal(size=(2, 9 * 5 * 5, 8))
+    x = rng.normal(size=(2, 9 * 5 * 5, U.IN_CH))
+    y = rng.normal(size=(2, 9 * 5 * 5, U.OUT_CH))
     loss, g, _ = r.loss_grad(x, y, p)
     offs = np.cumsum([0] + [27 * l[2] * l[5] + l[5] for l in r.L])
     checked = 0
