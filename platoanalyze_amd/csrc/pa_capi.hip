@@ -137,7 +137,7 @@ struct pa_problem {
   bool mg_fp32_smooth = true;
   int mg_pre0 = 1, mg_post0 = 1, mg_degc = 2;
   double mg_ratio = 60.0;
-  DevArray<double> g_x, g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
+  DevArray<double> g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -832,8 +832,7 @@ int pa_set_box(pa_handle h, int nx, int ny, int nz) {
   return PA_OK;
 }
 
-static int mg_level_init(pa_handle h, MgLevel& L, const std::vector<int32_t>& grid) {
-  (void)h;
+static int mg_level_init(MgLevel& L, const std::vector<int32_t>& grid) {
   PA_HIP(L.grid.upload(grid));
   std::vector<int32_t> nog((size_t)L.dims.nx * L.dims.ny * L.dims.nz, 0);
   for (int n = 0; n < L.nn; ++n)
@@ -873,7 +872,7 @@ static int mg_build(pa_handle h) {
   L0->h = h;
   L0->dims = h->box;
   L0->nn = h->nn;
-  if (int rc = mg_level_init(h, *L0, h->box_grid)) return rc;
+  if (int rc = mg_level_init(*L0, h->box_grid)) return rc;
   std::vector<int32_t> prev_grid = h->box_grid;
   h->mg.push_back(std::move(L0));
   while (true) {
@@ -915,7 +914,7 @@ static int mg_build(pa_handle h) {
     }
     if (int rc = pa_finalize(hc)) return rc;
     hc->stream = h->stream;
-    if (int rc = mg_level_init(h, *L, grid)) return rc;
+    if (int rc = mg_level_init(*L, grid)) return rc;
     PA_HIP(L->vals.alloc(9 * (size_t)hc->graph.rowmap[hc->nn]));
     prev_grid.swap(grid);
     h->mg.push_back(std::move(L));
@@ -1318,7 +1317,7 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
   }
   if (st) {
     st->iterations = it;
-    st->outer_iterations = mixed ? replacements : (int)h->mg.size();
+    st->outer_iterations = mixed ? replacements : 1;
     st->converged = conv ? 1 : 0;
     st->rel_residual = true_rel;
   }

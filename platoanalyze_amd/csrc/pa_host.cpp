@@ -375,10 +375,12 @@ Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, c
         while (i < tmp.size() && (tmp[i] >> 4) == el) mask |= (uint32_t)(tmp[i++] & 15);
         L.push_back((uint32_t)el | (mask << 28));
       }
-      // most-owned first, element order within a group: lanes of a wavefront then
-      // loop over the same number of owned vertices
+      // Group by owned-vertex mask (most-owned first, element order within a group):
+      // the lanes of a wavefront then skip the same element edges (those with no owned
+      // end), so a skipped edge costs nothing instead of running masked.
       std::stable_sort(L.begin(), L.end(), [](uint32_t x, uint32_t y) {
-        return __builtin_popcount(x >> 28) > __builtin_popcount(y >> 28);
+        const int px = __builtin_popcount(x >> 28), py = __builtin_popcount(y >> 28);
+        return px != py ? px > py : (x >> 28) < (y >> 28);
       });
       // Within each group, interleave with stride ceil(n/64): the 64 lanes of a
       // wavefront then take elements spread over the chunk instead of neighbours
@@ -387,8 +389,8 @@ Chunks build_chunks(const Graph& g, const NodeElems& adj, const int32_t* conn, c
       size_t g0 = 0;
       while (g0 < L.size()) {
         size_t g1 = g0;
-        const int pc = __builtin_popcount(L[g0] >> 28);
-        while (g1 < L.size() && __builtin_popcount(L[g1] >> 28) == pc) ++g1;
+        const uint32_t gm = L[g0] >> 28;
+        while (g1 < L.size() && (L[g1] >> 28) == gm) ++g1;
         const size_t n = g1 - g0, q = (n + 63) / 64;
         size_t k = 0;
         for (size_t col = 0; col < q; ++col)

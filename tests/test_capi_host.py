@@ -103,3 +103,51 @@ def test_error_paths():
         L.pa_destroy(h)
     with pytest.raises(_capi.PaError):
         _capi.call("pa_build_box", 0, 1, 1, 1.0, 1.0, 1.0, None, None, None, None)
+
+
+def _handle(mesh):
+    h = C.c_void_p()
+    coords = np.ascontiguousarray(mesh.coords, np.float64)
+    conn = np.ascontiguousarray(mesh.conn, np.int32)
+    _capi.call("pa_create", C.byref(h), mesh.num_nodes, mesh.num_elems, C.c_void_p(coords.ctypes.data),
+               C.c_void_p(conn.ctypes.data))
+    return h
+
+
+def test_set_box_accepts_lattices_in_any_node_order():
+    """pa_set_box locates every node on the (nx+1)(ny+1)(nz+1) lattice of the bounding
+    box (host logic: no device call before pa_finalize)."""
+    from platoanalyze_amd.mesh import TetMesh
+
+    m = build_box(4, 2, 3, 2.0, 1.0, 1.5)
+    h = _handle(m)
+    L = _capi.lib()
+    try:
+        assert L.pa_set_box(h, 4, 2, 3) == 0
+        assert L.pa_set_box(h, 4, 3, 2) == 3       # node count matches, lattice does not
+        assert L.pa_set_box(h, 4, 2, 4) == 3       # node count does not match
+        assert L.pa_set_box(h, 0, 2, 3) == 1       # bad argument
+    finally:
+        L.pa_destroy(h)
+    # a permuted node numbering is still the same lattice
+    perm = np.random.default_rng(5).permutation(m.num_nodes)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(m.num_nodes)
+    mp = TetMesh(m.coords[perm], inv[m.conn].astype(np.int32))
+    h = _handle(mp)
+    try:
+        assert L.pa_set_box(h, 4, 2, 3) == 0
+    finally:
+        L.pa_destroy(h)
+    # a node moved off the lattice, or two nodes on one lattice point
+    for bad in ("moved", "duplicate"):
+        c = m.coords.copy()
+        if bad == "moved":
+            c[7] += np.array([0.0, 0.1, 0.0])
+        else:
+            c[7] = c[8]
+        h = _handle(TetMesh(c, m.conn))
+        try:
+            assert L.pa_set_box(h, 4, 2, 3) == 3, bad
+        finally:
+            L.pa_destroy(h)
