@@ -127,3 +127,39 @@ def test_gpu_mg_errors():
     with pytest.raises(ValueError):
         Problem(m, preconditioner="amg")
     _ = solve
+
+
+@pytest.mark.slow
+def test_gpu_pnorm_adjoint_fd_128x64x64():
+    """The stress p-norm workload at full size (128x64x64 x 6 Tet4, smoothed density,
+    relaxed von Mises p = 8, q = 0.5): the adjoint gradient from the mixed-precision
+    multigrid solves (forward and adjoint) against central differences, h = 1e-6, at 64
+    seeded nodes, each side a FP64 multigrid solve to 1e-13.  Tolerance: 1e-6 of
+    max |dF/dz| plus 1e-4 of the difference itself (FD truncation and solve noise)."""
+    from platoanalyze_amd import criteria
+
+    print("case: p-norm adjoint vs FD at 128x64x64, 64 nodes", flush=True)
+    w = wl.make("cantilever_128x64x64")
+    m = wl.model(w)
+    pb = Problem(m, tolerance=1e-10, precision="mixed", preconditioner="multigrid")
+    z = dev(w.z).requires_grad_(True)
+    F = criteria.stress_pnorm(z, pb, 8.0, "von_mises", 0.5)
+    F.backward()
+    g = host(z.grad)
+    crit = criteria.StressPNorm(m, 8.0, "von_mises", 0.5)
+    ref = Problem(m, tolerance=1e-13, precision="fp64", preconditioner="multigrid")
+    nodes = np.random.default_rng(w.cfg.seed).choice(w.mesh.num_nodes, 64, replace=False)
+    h = 1e-6
+    worst = 0.0
+    for n in nodes:
+        vals = []
+        for s in (1.0, -1.0):
+            zz = w.z.copy()
+            zz[n] += s * h
+            dz = dev(zz)
+            vals.append(crit.value(ref.solution(dz), dz))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        err = abs(fd - g[n])
+        worst = max(worst, err / np.abs(g).max())
+        assert err <= 1e-6 * np.abs(g).max() + 1e-4 * abs(fd), (int(n), fd, g[n])
+    print(f"F {float(F.detach()):.10e}; worst |FD - adjoint| / max|g| = {worst:.2e} over 64 nodes", flush=True)
