@@ -117,6 +117,11 @@ int pa_copy_graph(pa_handle h, int32_t* rowmap, int32_t* colidx, int on_device);
 int pa_vf_value(pa_handle h, const double* u, const double* z, double* R);
 int pa_vf_gradient_u(pa_handle h, const double* u, const double* z, double* values, double* R);
 int pa_vf_gradient_z(pa_handle h, const double* u, const double* z, double* values_t);
+/* gradient_x: dR/dX as transposed 3x3 blocks on the same graph (9*nnz_b): block (config
+ * node b, residual node a), entry (k, i) = dR_{a,i}/dX_{b,k}, including the traction
+ * load's dependence on its faces (src/elliptic/VectorFunction.hpp:317-408, assembled
+ * by assembleTransposeJacobian, src/Assembly.hpp:1422-1485) */
+int pa_vf_gradient_x(pa_handle h, const double* u, const double* z, double* values_t);
 
 /* ---------------------------------- Plato::Elliptic::PhysicsScalarFunction, criterion
  * "Internal Elastic Energy" (src/elliptic/InternalElasticEnergy.hpp:100-158):
@@ -126,6 +131,9 @@ int pa_vf_gradient_z(pa_handle h, const double* u, const double* z, double* valu
 int pa_sf_value(pa_handle h, const double* u, const double* z, double* value);
 int pa_sf_gradient_u(pa_handle h, const double* u, const double* z, double* g);
 int pa_sf_gradient_z(pa_handle h, const double* u, const double* z, double* g, double* value);
+/* gradient_x = partial dC/dX at fixed u and z (3N)
+ *                                       (PhysicsScalarFunction.hpp:293-355) */
+int pa_sf_gradient_x(pa_handle h, const double* u, const double* z, double* g);
 
 /* ------------------------------------------------------------ Problem / solver (device) */
 typedef struct {

@@ -23,6 +23,10 @@
 //   orc_apply_constraints / orc_pcg .. dense LAPACK solve on small meshes (numpy)
 //   orc_traction ..................... total force = area * traction (closed form)
 //   orc_stress_pnorm ................. DerivativeTests.cpp:1058 (14525.25169157 at p=12)
+//   orc_gradient_x ................... DerivativeTests.cpp:469-483 (36 entries); central
+//                                      differences of orc_residual in the coordinates
+//   orc_energy_gradient_x ............ DerivativeTests.cpp:665-693 (81 values); central
+//                                      differences of orc_energy in the coordinates
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -117,6 +121,15 @@ inline Fad<N> fad_sqrt(const Fad<N>& a) {
 inline double fad_sqrt(double a) { return std::sqrt(a); }
 template <int N>
 inline Fad<N> operator*(const Fad<N>& a, int b) { return a * (double)b; }
+template <int N>
+inline Fad<N> operator/(const Fad<N>& a, const Fad<N>& b) {
+  Fad<N> r(a.v / b.v);
+  for (int i = 0; i < N; ++i) r.dx[i] = (a.dx[i] * b.v - a.v * b.dx[i]) / (b.v * b.v);
+  return r;
+}
+template <int N>
+inline Fad<N> fad_abs(const Fad<N>& a) { return a.v < 0.0 ? a * -1.0 : a; }
+inline double fad_abs(double a) { return std::fabs(a); }
 // product of a stress-typed and a weight-typed value into the result type
 template <typename R, typename A, typename B>
 inline R mix(const A& a, const B& b) { return R(a * b); }
@@ -239,6 +252,53 @@ inline int block_entry(const int* rowmap, const int* col, int row, int colnode) 
 inline void atomic_add(double* p, double v) {
 #pragma omp atomic
   *p += v;
+}
+
+// Configuration-typed element residual and energy for the shape derivatives
+// (VectorFunction::gradient_x, src/elliptic/VectorFunction.hpp:317-408, and
+// PhysicsScalarFunction::gradient_x, src/elliptic/PhysicsScalarFunction.hpp:293-355):
+// the same gradient, strain, stress, weighting and divergence arithmetic with the
+// cell coordinates as dual numbers (GradientX: ConfigFad = SFad<double,12>).
+template <typename T>
+void gradient_workset_t(const T* cfg /*4x3*/, T* grad /*4x3*/, T& vol) {
+  auto C = [&](int n, int d) { return cfg[3 * n + d]; };
+  T j11 = C(0, 0) - C(3, 0), j12 = C(1, 0) - C(3, 0), j13 = C(2, 0) - C(3, 0);
+  T j21 = C(0, 1) - C(3, 1), j22 = C(1, 1) - C(3, 1), j23 = C(2, 1) - C(3, 1);
+  T j31 = C(0, 2) - C(3, 2), j32 = C(1, 2) - C(3, 2), j33 = C(2, 2) - C(3, 2);
+  T detj = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 - j12 * j21 * j33 -
+           j13 * j22 * j31;
+  grad[0] = (j22 * j33 - j23 * j32) / detj;
+  grad[1] = (j13 * j32 - j12 * j33) / detj;
+  grad[2] = (j12 * j23 - j13 * j22) / detj;
+  grad[3] = (j23 * j31 - j21 * j33) / detj;
+  grad[4] = (j11 * j33 - j13 * j31) / detj;
+  grad[5] = (j13 * j21 - j11 * j23) / detj;
+  grad[6] = (j21 * j32 - j22 * j31) / detj;
+  grad[7] = (j12 * j31 - j11 * j32) / detj;
+  grad[8] = (j11 * j22 - j12 * j21) / detj;
+  for (int d = 0; d < 3; ++d) grad[9 + d] = (grad[d] + grad[3 + d] + grad[6 + d]) * -1.0;
+  vol = fad_abs(detj) * (1.0 / 2.0 / 3.0);
+}
+
+template <typename T>
+void element_stress_t(const Params& P, const T* g, const double* u, double w, T* eps, T* sw) {
+  int t = 0;
+  for (int d = 0; d < 3; ++d) {
+    eps[t] = T(0.0);
+    for (int n = 0; n < 4; ++n) eps[t] += g[3 * n + d] * u[3 * n + d];
+    ++t;
+  }
+  for (int J = 2; J >= 1; --J)
+    for (int I = J - 1; I >= 0; --I) {
+      eps[t] = T(0.0);
+      for (int n = 0; n < 4; ++n) eps[t] += g[3 * n + I] * u[3 * n + J] + g[3 * n + J] * u[3 * n + I];
+      ++t;
+    }
+  for (int i = 0; i < 6; ++i) {
+    T si(0.0);
+    for (int j = 0; j < 6; ++j) si += eps[j] * P.c[i][j];
+    sw[i] = si * w;
+  }
 }
 
 struct Mesh {
@@ -803,6 +863,99 @@ void orc_dRdz_T_apply(int nn, int ne, const double* X, const int* conn, const do
       for (int k = 0; k < 12; ++k) acc += le[k] * r[k].dx[n];
       atomic_add(&out[conn[4 * e + n]], acc);
     }
+  }
+}
+
+// VectorFunction::gradient_x: dR/dX with the configuration seeded as SFad<12>, stored
+// transposed like gradient_z (assemble_transpose_jacobian, src/Assembly.hpp:1422-1485):
+// block (config node b, residual node a), entry (k, i) = dR_{a,i}/dX_{b,k}; then the
+// boundary pass: the traction load's dependence on the face coordinates (SFad over the
+// face vertices; R = K u - f).
+void orc_gradient_x(int nn, int ne, const double* X, const int* conn, const double* u, const double* z, double E,
+                    double nu, double p, double rmin, int nfaces, const int* face_verts, const double* t,
+                    const int* rowmap, const int* col, double* values_t) {
+  using F = Fad<12>;
+  Mesh m{nn, ne, X, conn};
+  Params P;
+  make_params(E, nu, p, rmin, P);
+  std::memset(values_t, 0, sizeof(double) * 9 * (size_t)rowmap[nn]);
+  auto cfgw = workset_config(m);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < ne; ++e) {
+    F cfg[12], g[12], vol;
+    double ue[12], zc[4];
+    for (int k = 0; k < 12; ++k) cfg[k] = F(cfgw[12 * (size_t)e + k], k);
+    for (int n = 0; n < 4; ++n) {
+      zc[n] = z[conn[4 * e + n]];
+      for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
+    }
+    gradient_workset_t(cfg, g, vol);
+    const double w = msimp_weight(P, zc);
+    F eps[6], sw[6], r[12];
+    element_stress_t(P, g, ue, w, eps, sw);
+    for (int i = 0; i < 3; ++i)
+      for (int n = 0; n < 4; ++n) {
+        F acc(0.0);
+        for (int j = 0; j < 3; ++j) acc += vol * sw[kVoigt[i][j]] * g[3 * n + j];
+        r[3 * n + i] = acc;
+      }
+    for (int row = 0; row < 12; ++row)
+      for (int cc = 0; cc < 12; ++cc) {
+        const int s2 = block_entry(rowmap, col, conn[4 * e + cc / 3], conn[4 * e + row / 3]);
+        atomic_add(&values_t[9 * (size_t)s2 + 3 * (cc % 3) + row % 3], r[row].dx[cc]);
+      }
+  }
+  const double cub = 1.0 / 2.0 / 3.0;
+  for (int f = 0; f < nfaces; ++f) {
+    using G = Fad<9>;
+    const int* v = face_verts + 3 * f;
+    G x[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int d = 0; d < 3; ++d) x[a][d] = G(X[3 * (size_t)v[a] + d], 3 * a + d);
+    G j0[3], j1[3];
+    for (int d = 0; d < 3; ++d) {
+      j0[d] = x[0][d] - x[2][d];
+      j1[d] = x[1][d] - x[2][d];
+    }
+    G c23 = j0[1] * j1[2] - j0[2] * j1[1];
+    G c31 = j0[2] * j1[0] - j0[0] * j1[2];
+    G c12 = j0[0] * j1[1] - j0[1] * j1[0];
+    G wf = fad_sqrt(c23 * c23 + c31 * c31 + c12 * c12) * (-1.0 * cub);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const int s2 = block_entry(rowmap, col, v[b], v[a]);
+        for (int k = 0; k < 3; ++k)
+          for (int i = 0; i < 3; ++i) values_t[9 * (size_t)s2 + 3 * k + i] += wf.dx[3 * b + k] * t[i];
+      }
+  }
+}
+
+// PhysicsScalarFunction::gradient_x of the internal elastic energy: d/dX of
+// sum_e 0.5 vol sigma_w . eps with the configuration seeded as SFad<12>; g (3N).
+void orc_energy_gradient_x(int nn, int ne, const double* X, const int* conn, const double* u, const double* z,
+                           double E, double nu, double p, double rmin, double* gx) {
+  using F = Fad<12>;
+  Mesh m{nn, ne, X, conn};
+  Params P;
+  make_params(E, nu, p, rmin, P);
+  std::memset(gx, 0, sizeof(double) * 3 * (size_t)nn);
+  auto cfgw = workset_config(m);
+#pragma omp parallel for schedule(static)
+  for (int e = 0; e < ne; ++e) {
+    F cfg[12], g[12], vol;
+    double ue[12], zc[4];
+    for (int k = 0; k < 12; ++k) cfg[k] = F(cfgw[12 * (size_t)e + k], k);
+    for (int n = 0; n < 4; ++n) {
+      zc[n] = z[conn[4 * e + n]];
+      for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
+    }
+    gradient_workset_t(cfg, g, vol);
+    const double w = msimp_weight(P, zc);
+    F eps[6], sw[6], inc(0.0);
+    element_stress_t(P, g, ue, w, eps, sw);
+    for (int k = 0; k < 6; ++k) inc += sw[k] * eps[k];
+    F val = inc * vol * 0.5;
+    for (int k = 0; k < 12; ++k) atomic_add(&gx[3 * (size_t)conn[4 * e + k / 3] + k % 3], val.dx[k]);
   }
 }
 

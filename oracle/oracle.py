@@ -98,6 +98,21 @@ class Model:
         lib().orc_energy(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), C.byref(val), _p(gu), _p(gz))
         return val.value, gu, gz
 
+    def gradient_x(self, u, z, rowmap, colidx):
+        """dR/dX, transposed 3x3 blocks: block (config node b, residual node a), entry
+        (k, i) = dR_{a,i}/dX_{b,k} (includes the traction's dependence on the faces)."""
+        v = np.empty(9 * int(rowmap[-1]))
+        nf = 0 if self.fv is None else self.fv.shape[0]
+        lib().orc_gradient_x(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), nf, _p(self.fv), _p(self.t),
+                             _p(_i32(rowmap)), _p(_i32(colidx)), _p(v))
+        return v
+
+    def energy_gradient_x(self, u, z):
+        """d/dX of the internal elastic energy at fixed u and z (3N)."""
+        g = np.empty(3 * self.nn)
+        lib().orc_energy_gradient_x(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), _p(g))
+        return g
+
     def cell_worksets(self, u, ncells):
         g = np.empty((ncells, 4, 3))
         vol = np.empty(ncells)

@@ -62,9 +62,11 @@ _SIGS = {
     "pa_vf_value": (_I, [_P, _P, _P, _P]),
     "pa_vf_gradient_u": (_I, [_P, _P, _P, _P, _P]),
     "pa_vf_gradient_z": (_I, [_P, _P, _P, _P]),
+    "pa_vf_gradient_x": (_I, [_P, _P, _P, _P]),
     "pa_sf_value": (_I, [_P, _P, _P, _PD]),
     "pa_sf_gradient_u": (_I, [_P, _P, _P, _P]),
     "pa_sf_gradient_z": (_I, [_P, _P, _P, _P, _PD]),
+    "pa_sf_gradient_x": (_I, [_P, _P, _P, _P]),
     "pa_apply_constraints": (_I, [_P, _P, _P, _D]),
     "pa_solve": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "pa_solution": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),

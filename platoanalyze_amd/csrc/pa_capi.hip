@@ -87,6 +87,8 @@ struct pa_problem {
   std::vector<int32_t> bc_dofs;
   std::vector<double> bc_vals;
   std::vector<double> f;  // dense load, filled by pa_add_traction
+  std::vector<int32_t> trac_fv;  // traction faces (3 vertices each) and their vectors,
+  std::vector<double> trac_t;    // kept for the load's shape derivative
   bool user_graph = false;
   pa::Graph graph;
   bool finalized = false;
@@ -120,6 +122,11 @@ struct pa_problem {
   DevArray<double> chunk_partials;
   // solver workspace (lazily allocated)
   DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
+  // shape derivative of the traction load: value positions in the transposed dR/dX
+  // matrix and their (constant) contributions, built on first use
+  bool gx_built = false;
+  DevArray<int64_t> gx_idx;
+  DevArray<double> gx_val;
 
   // geometric multigrid (box meshes, pa_set_box)
   bool has_box = false;
@@ -327,6 +334,8 @@ int pa_add_traction(pa_handle h, int num_faces, const int32_t* fv, const int32_t
     const double w = cub * std::sqrt(j23 * j23 + j31 * j31 + j12 * j12);
     for (int a = 0; a < 3; ++a)
       for (int d = 0; d < 3; ++d) h->f[3 * (size_t)v[a] + d] += w * t[d];
+    h->trac_fv.insert(h->trac_fv.end(), v, v + 3);
+    h->trac_t.insert(h->trac_t.end(), t, t + 3);
   }
   return PA_OK;
 }
@@ -362,7 +371,7 @@ int pa_finalize(pa_handle h) {
   if (const char* env = std::getenv("PA_CHUNK_NODES")) max_nodes = std::max(64, std::min(4096, std::atoi(env)));
   const int wpc = h->code_bits == 4 ? 2 : 4;
   h->chunks = pa::build_chunks(g, adj, h->conn.data(), codes, wpc, max_blocks, max_rows, max_nodes);
-  for (int mode = 0; mode <= pa::kModeAdjZ; ++mode)
+  for (int mode = 0; mode < pa::kNumChunkModes; ++mode)
     if (pa::chunk_lds_bytes(mode, h->chunks.max_rows, h->chunks.max_blocks, h->chunks.max_nodes) > 160 * 1024)
       return fail(PA_ERR_UNSUPPORTED, "chunk limits exceed the 160 KiB of LDS of a work-group");
   if (h->chunks.max_blocks > max_blocks || h->chunks.max_nodes > max_nodes)
@@ -488,6 +497,82 @@ int pa_vf_gradient_z(pa_handle h, const double* u, const double* z, double* vals
   if (int rc = ready(h)) return rc;
   if (!u || !z || !vals_t) return fail(PA_ERR_ARG, "pa_vf_gradient_z: null pointer");
   return run_chunk(h, pa::kModeGradZT, u, z, vals_t, nullptr);
+}
+
+// Traction part of dR/dX: R = K u - f with f_a = (|c|/6) t on each face vertex a,
+// c = (x0 - x2) x (x1 - x2); d|c|/dx0 = j1 x c^, d|c|/dx1 = c^ x j0, d|c|/dx2 = -(both).
+// Contributions to block (config vertex b, residual vertex a), entry (k, i), summed per
+// position on the host (they depend on the coordinates and loads only).
+static int build_gx_traction(pa_handle h) {
+  if (h->gx_built) return PA_OK;
+  const pa::Graph& g = h->graph;
+  std::vector<std::pair<int64_t, double>> pv;
+  const size_t nf = h->trac_fv.size() / 3;
+  const double* c = h->coords.data();
+  for (size_t f = 0; f < nf; ++f) {
+    const int32_t* v = &h->trac_fv[3 * f];
+    const double* t = &h->trac_t[3 * f];
+    double j0[3], j1[3];
+    for (int d = 0; d < 3; ++d) {
+      j0[d] = c[3 * (size_t)v[0] + d] - c[3 * (size_t)v[2] + d];
+      j1[d] = c[3 * (size_t)v[1] + d] - c[3 * (size_t)v[2] + d];
+    }
+    const double cx[3] = {j0[1] * j1[2] - j0[2] * j1[1], j0[2] * j1[0] - j0[0] * j1[2], j0[0] * j1[1] - j0[1] * j1[0]};
+    const double nrm = std::sqrt(cx[0] * cx[0] + cx[1] * cx[1] + cx[2] * cx[2]);
+    if (!(nrm > 0.0)) continue;
+    const double n[3] = {cx[0] / nrm, cx[1] / nrm, cx[2] / nrm};
+    double dA[3][3];  // d|c| / dx_b
+    dA[0][0] = j1[1] * n[2] - j1[2] * n[1];
+    dA[0][1] = j1[2] * n[0] - j1[0] * n[2];
+    dA[0][2] = j1[0] * n[1] - j1[1] * n[0];
+    dA[1][0] = n[1] * j0[2] - n[2] * j0[1];
+    dA[1][1] = n[2] * j0[0] - n[0] * j0[2];
+    dA[1][2] = n[0] * j0[1] - n[1] * j0[0];
+    for (int k = 0; k < 3; ++k) dA[2][k] = -(dA[0][k] + dA[1][k]);
+    const double cub = 1.0 / 2.0 / 3.0;
+    for (int b = 0; b < 3; ++b)
+      for (int a = 0; a < 3; ++a) {
+        int slot = -1;
+        for (int q = g.rowmap[v[b]]; q < g.rowmap[v[b] + 1]; ++q)
+          if (g.colidx[q] == v[a]) slot = q;
+        if (slot < 0) return fail(PA_ERR_MESH, "traction face edge missing from the graph");
+        for (int k = 0; k < 3; ++k)
+          for (int i = 0; i < 3; ++i) pv.emplace_back(9 * (int64_t)slot + 3 * k + i, -cub * dA[b][k] * t[i]);
+      }
+  }
+  std::sort(pv.begin(), pv.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<int64_t> idx;
+  std::vector<double> val;
+  for (size_t k = 0; k < pv.size(); ++k) {
+    if (!idx.empty() && idx.back() == pv[k].first) val.back() += pv[k].second;
+    else {
+      idx.push_back(pv[k].first);
+      val.push_back(pv[k].second);
+    }
+  }
+  PA_HIP(h->gx_idx.upload(idx));
+  PA_HIP(h->gx_val.upload(val));
+  h->gx_built = true;
+  return PA_OK;
+}
+
+int pa_vf_gradient_x(pa_handle h, const double* u, const double* z, double* vals_t) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !z || !vals_t) return fail(PA_ERR_ARG, "pa_vf_gradient_x: null pointer");
+  if (int rc = build_gx_traction(h)) return rc;
+  if (int rc = run_chunk(h, pa::kModeGradX, u, z, vals_t, nullptr)) return rc;
+  const size_t n = h->gx_idx.n;
+  if (n) {
+    pa::k_scatter_add<<<grid_for(n), 256, 0, h->stream>>>(vals_t, h->gx_idx.p, h->gx_val.p, n);
+    PA_HIP(hipGetLastError());
+  }
+  return PA_OK;
+}
+
+int pa_sf_gradient_x(pa_handle h, const double* u, const double* z, double* g) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !z || !g) return fail(PA_ERR_ARG, "pa_sf_gradient_x: null pointer");
+  return run_chunk(h, pa::kModeEnergyGradX, u, z, nullptr, g);
 }
 
 // --------------------------------------------------------------- scalar function

@@ -107,9 +107,10 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   constexpr bool kGradZ = MODE == kModeEnergyGradZ;
   constexpr bool kPnV = MODE == kModePNorm, kPnG = MODE == kModePNormGrad;
   constexpr bool kAdj = MODE == kModeAdjZ;
+  constexpr bool kGx = MODE == kModeGradX, kEGx = MODE == kModeEnergyGradX;
   constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV;
-  constexpr int kBlkW = kGzT ? 3 : 0;                                  // doubles per block in LDS
-  constexpr int kRowW = (kRes || kPnG) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row in LDS
+  constexpr int kBlkW = kGzT ? 3 : (kGx ? 9 : 0);                      // doubles per block in LDS
+  constexpr int kRowW = (kRes || kPnG || kEGx) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row in LDS
   constexpr int kRow2 = kPnG ? 1 : 0;                                  // second per-row output
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
@@ -303,6 +304,84 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
           const int sl = base + (int)((code >> (BITS * b)) & ((1u << BITS) - 1));
 #pragma unroll
           for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], dw * rb[b][i]);
+        }
+      }
+    }
+    if constexpr (kGx || kEGx) {
+      // Shape derivatives.  With H = sum_b u_b g_b^T, sigma_w = w C eps (3x3 tensor S),
+      // dvol/dX_{b,k} = vol g_bk, dg_aj/dX_{b,k} = -g_bj g_ak, dH/dX_{b,k} = -H[:,k] g_b^T:
+      //   energy: dE/dX_{a,k} = vol (psi g_ak - sum_p H_pk (S g_a)_p), psi = sigma_w.eps / 2
+      //   residual r_a = vol S g_a:
+      //   dr_{a,i}/dX_{b,k} = vol (g_bk (S g_a)_i - (S g_b)_i g_ak + (dS_bk g_a)_i),
+      //   dS_bk = w C sym(dH/dX_{b,k}).
+      double H[3][3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) H[p][q] = u[0][p] * g[0][q] + u[1][p] * g[1][q] + u[2][p] * g[2][q] + u[3][p] * g[3][q];
+      const double S[3][3] = {{w * s[0], w * s[5], w * s[4]}, {w * s[5], w * s[1], w * s[3]}, {w * s[4], w * s[3], w * s[2]}};
+      double Sg[4][3];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Sg[a][i] = S[i][0] * g[a][0] + S[i][1] * g[a][1] + S[i][2] * g[a][2];
+      if constexpr (kEGx) {
+        double psi = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) psi += (w * s[q]) * eps[q];
+        psi *= 0.5;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+          if (ln[a] >= nr) continue;
+          double* dst = &res[3 * ln[a]];
+#pragma unroll
+          for (int k2 = 0; k2 < 3; ++k2)
+            lds_add(dst + k2, vol * (psi * g[a][k2] - (H[0][k2] * Sg[a][0] + H[1][k2] * Sg[a][1] + H[2][k2] * Sg[a][2])));
+        }
+      } else {
+        uint32_t cw[4];
+        if constexpr (BITS == 4) {
+          const uint2 t = reinterpret_cast<const uint2*>(A.ent_code)[k];
+          cw[0] = t.x;
+          cw[1] = t.y;
+          cw[2] = t.y;
+          cw[3] = t.y;
+        } else {
+          const uint4 t = reinterpret_cast<const uint4*>(A.ent_code)[k];
+          cw[0] = t.x;
+          cw[1] = t.y;
+          cw[2] = t.z;
+          cw[3] = t.w;
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {  // config vertex b = the chunk's row
+          if (ln[b] >= nr) continue;
+          const uint32_t code = BITS == 4 ? (cw[b >> 1] >> (16 * (b & 1))) : cw[b];
+          const int base = roff[ln[b]];
+#pragma unroll
+          for (int k2 = 0; k2 < 3; ++k2) {
+            // dH = -H[:,k] g_b^T; Voigt strain with engineering shear; dS = w C deps
+            double de[6];
+            de[0] = -H[0][k2] * g[b][0];
+            de[1] = -H[1][k2] * g[b][1];
+            de[2] = -H[2][k2] * g[b][2];
+            de[3] = -(H[1][k2] * g[b][2] + H[2][k2] * g[b][1]);
+            de[4] = -(H[0][k2] * g[b][2] + H[2][k2] * g[b][0]);
+            de[5] = -(H[0][k2] * g[b][1] + H[1][k2] * g[b][0]);
+            double ds[6];
+            iso_stress(A.m, de, ds);
+            const double dS[3][3] = {{w * ds[0], w * ds[5], w * ds[4]}, {w * ds[5], w * ds[1], w * ds[3]},
+                                     {w * ds[4], w * ds[3], w * ds[2]}};
+#pragma unroll
+            for (int a = 0; a < 4; ++a) {
+              const int sl = base + (int)((code >> (BITS * a)) & ((1u << BITS) - 1));
+#pragma unroll
+              for (int i = 0; i < 3; ++i)
+                lds_add(&seg[9 * sl + 3 * k2 + i],
+                        vol * (g[b][k2] * Sg[a][i] - Sg[b][i] * g[a][k2] +
+                               (dS[i][0] * g[a][0] + dS[i][1] * g[a][1] + dS[i][2] * g[a][2])));
+            }
+          }
         }
       }
     }
@@ -609,8 +688,8 @@ void k_jac_chunk(ChunkArgs A) {
 
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
   if (mode == kModeJacobian || mode == kModeJacobianResidual) return jac_lds_layout(max_rows, max_blocks, max_nodes).total;
-  const int blkw = mode == kModeGradZT ? 3 : 0;
-  const int roww = (mode == kModeResidual || mode == kModePNormGrad) ? 3
+  const int blkw = mode == kModeGradZT ? 3 : (mode == kModeGradX ? 9 : 0);
+  const int roww = (mode == kModeResidual || mode == kModePNormGrad || mode == kModeEnergyGradX) ? 3
                    : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
   const int row2 = mode == kModePNormGrad ? 1 : 0;
   const int nodew = mode == kModeAdjZ ? 10 : 7;
@@ -639,6 +718,8 @@ static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
     case kModePNorm: k_chunk<BITS, kModePNorm, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModePNormGrad: k_chunk<BITS, kModePNormGrad, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeAdjZ: k_chunk<BITS, kModeAdjZ, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeGradX: k_chunk<BITS, kModeGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeEnergyGradX: k_chunk<BITS, kModeEnergyGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -660,6 +741,8 @@ hipError_t launch_chunk(int bits, int mode, const ChunkArgs& a, hipStream_t s) {
     allow_lds(&k_jac_chunk<8, true, kNtJac>);
     allow_lds(&k_chunk<4, kModeGradZT, kNtVec>);
     allow_lds(&k_chunk<8, kModeGradZT, kNtVec>);
+    allow_lds(&k_chunk<4, kModeGradX, kNtVec>);
+    allow_lds(&k_chunk<8, kModeGradX, kNtVec>);
   }
   if (a.nchunks == 0) return hipSuccess;
   return bits == 4 ? launch_chunk_b<4>(mode, a, s) : launch_chunk_b<8>(mode, a, s);

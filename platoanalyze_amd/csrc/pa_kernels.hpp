@@ -65,7 +65,8 @@ struct ChunkArgs {
 };
 
 enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
-                 kModeEnergy = 4, kModeEnergyGradZ = 5, kModePNorm = 6, kModePNormGrad = 7, kModeAdjZ = 8 };
+                 kModeEnergy = 4, kModeEnergyGradZ = 5, kModePNorm = 6, kModePNormGrad = 7, kModeAdjZ = 8,
+                 kModeGradX = 9, kModeEnergyGradX = 10, kNumChunkModes = 11 };
 
 
 // launch helpers (defined in pa_kernels.hip)

@@ -203,6 +203,10 @@ __global__ __launch_bounds__(kT) void k_add_load(double* y, const int* nodes, co
   }
 }
 
+__global__ __launch_bounds__(kT) void k_scatter_add(double* y, const int64_t* idx, const double* v, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[idx[i]] += v[i];
+}
+
 __global__ __launch_bounds__(kT) void k_fill(double* y, size_t n, double v) {
   for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) y[i] = v;
 }

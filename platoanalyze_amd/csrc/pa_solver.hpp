@@ -27,6 +27,8 @@ __global__ void k_sum(const double* partials, int n, double* scal, int slot);
 __global__ void k_diag_inv(const double* vals, const int* diag_slot, double* dinv, int nnodes);
 __global__ void k_add_load(double* y, const int* nodes, const double* f, int n, double s);
 __global__ void k_fill(double* y, size_t n, double v);
+// y[idx[k]] += v[k] (distinct positions)
+__global__ void k_scatter_add(double* y, const int64_t* idx, const double* v, size_t n);
 __global__ void k_scale(double* y, size_t n, double s);
 __global__ void k_axpy(double* y, const double* x, size_t n, double a);
 __global__ void k_bc_lift(const double* vals, const int* rowmap, const int* col, const int* dofs,

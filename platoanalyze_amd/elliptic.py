@@ -175,6 +175,14 @@ class VectorFunction:
         _capi.call("pa_vf_gradient_z", self.m.handle, _dptr(u), _dptr(z), _dptr(vals))
         return CrsMatrix(self.m.row_map, self.m.column_indices, vals, 1, 3)
 
+    def gradient_x(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> CrsMatrix:
+        """dR/dX, transposed 3x3 blocks: block (config node, residual node)."""
+        self._check(u, z)
+        vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+        self.m._stream()
+        _capi.call("pa_vf_gradient_x", self.m.handle, _dptr(u), _dptr(z), _dptr(vals))
+        return CrsMatrix(self.m.row_map, self.m.column_indices, vals, 3, 3)
+
 
 class PhysicsScalarFunction:
     """Plato::Elliptic::PhysicsScalarFunction for "Internal Elastic Energy"."""
@@ -200,6 +208,13 @@ class PhysicsScalarFunction:
         g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=u.device)
         self.m._stream()
         _capi.call("pa_sf_gradient_z", self.m.handle, _dptr(u), _dptr(z), _dptr(g), None)
+        return g
+
+    def gradient_x(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
+        """Partial derivative of the energy in the nodal coordinates (3N)."""
+        g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        self.m._stream()
+        _capi.call("pa_sf_gradient_x", self.m.handle, _dptr(u), _dptr(z), _dptr(g))
         return g
 
 

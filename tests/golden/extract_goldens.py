@@ -51,11 +51,13 @@ def main(ref: Path) -> None:
     out["cells3d"] = {k: array(b, k) for k in ("gradient_gold", "strain_gold", "stress_gold", "result_gold")}
     b = test_body(src, "ElastostaticResidual3D")
     out["elastostatic_residual3d"] = {k: array(b, k)
-                                      for k in ("residual_gold", "gold_jac_entries", "gold_grad_entries")}
+                                      for k in ("residual_gold", "gold_jac_entries", "gold_grad_entries",
+                                                "gold_grad_x_entries")}
     b = test_body(src, "InternalElasticEnergy3D")
     out["internal_elastic_energy3d"] = {"value_gold": scalar(b, "value_gold"),
                                         "grad_u_gold": array(b, "grad_u_gold"),
-                                        "grad_z_gold": array(b, "grad_z_gold")}
+                                        "grad_z_gold": array(b, "grad_z_gold"),
+                                        "grad_x_gold": array(b, "grad_x_gold")}
     b = test_body(src, "StressPNorm3D")
     out["stress_pnorm3d"] = {"value_gold": scalar(b, "value_gold"), "grad_u_gold": array(b, "grad_u_gold"),
                              "grad_z_gold": array(b, "grad_z_gold")}

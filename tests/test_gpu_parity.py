@@ -394,3 +394,46 @@ def test_gpu_pnorm_total_gradient_fd_and_autograd(precision):
         print(f"node {n}: FD {fd:.10e} adjoint {g[n]:.10e}", flush=True)
         assert abs(fd - g[n]) <= 1e-6 * np.abs(g).max() + 1e-7 * abs(fd), (n, fd, g[n])
     assert float(C) > 0
+
+
+# ------------------------------------------------------------------ shape derivatives
+def test_gpu_gradient_x_goldens_2x2x2(box222, goldens):
+    """VectorFunction::gradient_x (DerivativeTests.cpp:469-486, 36 entries, transposed
+    blocks in the reference's row-0 order) and PhysicsScalarFunction::gradient_x of the
+    internal elastic energy (:660-693, 81 values)."""
+    print("case: 2x2x2 gradient_x goldens", flush=True)
+    rm, ci = orc.graph(box222.conn, 27)
+    ci = ci.copy()
+    ci[rm[0]:rm[1]] = REF_ROW0
+    m = ElasticModel(box222, 1e6, 0.3, 1.0, 0.0, graph=(rm, ci))
+    u, z = dev(ref_state(27)), dev(np.ones(27))
+    G = host(VectorFunction(m).gradient_x(u, z).entries)
+    gold = np.array(goldens["elastostatic_residual3d"]["gold_grad_x_entries"])
+    np.testing.assert_allclose(G[:36], gold, rtol=1e-13, atol=1e-9)
+    gx = host(PhysicsScalarFunction(m).gradient_x(u, z))
+    close(gx, goldens["internal_elastic_energy3d"]["grad_x_gold"], 1e-12, "energy grad_x")
+
+
+def test_gpu_gradient_x_cfg0(cfg0):
+    """dR/dX (with the traction's face dependence) and d(energy)/dX against the oracle's
+    dual-number evaluation on config 0."""
+    print("case: cfg0 gradient_x", flush=True)
+    w, m, o, rm, ci, u, z = cfg0
+    du, dz = dev(u), dev(z)
+    close(host(VectorFunction(m).gradient_x(du, dz).entries), o.gradient_x(u, z, rm, ci), RTOL, "dR/dX")
+    close(host(PhysicsScalarFunction(m).gradient_x(du, dz)), o.energy_gradient_x(u, z), RTOL, "dE/dX")
+
+
+@pytest.mark.parametrize("seed", [0])
+def test_gpu_gradient_x_unstructured(seed):
+    """Unstructured tets with rows wider than 16 blocks (8-bit slot codes)."""
+    mesh = _random_delaunay(400, seed)
+    rm, ci = orc.graph(mesh.conn, mesh.num_nodes)
+    m = ElasticModel(mesh, 2.0, 0.25, 3.0, 1e-3)
+    o = orc.Model(mesh.coords, mesh.conn, 2.0, 0.25, 3.0, 1e-3)
+    rng = np.random.default_rng(seed + 20)
+    u = rng.normal(size=3 * mesh.num_nodes)
+    z = rng.uniform(0.05, 1.0, mesh.num_nodes)
+    du, dz = dev(u), dev(z)
+    close(host(VectorFunction(m).gradient_x(du, dz).entries), o.gradient_x(u, z, rm, ci), RTOL, "dR/dX")
+    close(host(PhysicsScalarFunction(m).gradient_x(du, dz)), o.energy_gradient_x(u, z), RTOL, "dE/dX")

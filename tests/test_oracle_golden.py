@@ -124,6 +124,21 @@ def test_energy_golden(ref_model, goldens):
     assert_close(gz, g["grad_z_gold"], 1e-13)
 
 
+def test_gradient_x_golden(ref_model, box222, goldens):
+    """ElastostaticResidual3D gradient_x (:469-486): transposed 3x3 blocks, first 36
+    entries, tol 1e-13 as the reference."""
+    rm, ci = reference_graph(box222.conn, 27)
+    G = ref_model.gradient_x(ref_state(27), np.ones(27), rm, ci)
+    assert_close(G[:36], goldens["elastostatic_residual3d"]["gold_grad_x_entries"], 1e-13)
+
+
+def test_energy_gradient_x_golden(ref_model, goldens):
+    """InternalElasticEnergy3D gradient_x (:660-693), 81 values, tol 1e-13 as the reference
+    (gold printed with 15 significant digits: 1e-12 where it is short)."""
+    g = ref_model.energy_gradient_x(ref_state(27), np.ones(27))
+    assert_close(g, goldens["internal_elastic_energy3d"]["grad_x_gold"], 1e-12)
+
+
 def test_stress_pnorm_golden(ref_model, goldens):
     """StressPNorm3D value (:1058), exponent 12, and its partial gradients (:1069-1108)."""
     g = goldens["stress_pnorm3d"]
@@ -219,6 +234,34 @@ def test_gradient_z_fd(small):
         for s in range(rm[n], rm[n + 1]):
             col[3 * ci[s]:3 * ci[s] + 3] = G[3 * s:3 * s + 3]
         np.testing.assert_allclose(col, fd, atol=1e-6 * np.abs(fd).max())
+
+
+def test_gradient_x_fd_with_traction(small):
+    """dR/dX (transposed blocks, including the traction's dependence on the loaded faces)
+    and d(energy)/dX against central differences of the residual and energy in the nodal
+    coordinates; the loaded nodes are among the probed ones."""
+    m, z, u, _, rm, ci = small
+    fv, fe = faces_on_plane(m, 0, 2.0)
+    t = np.array([0.3, -0.2, -1.0])
+    model = orc.Model(m.coords, m.conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3, face_verts=fv, traction=t)
+    G = model.gradient_x(u, z, rm, ci)
+    gE = model.energy_gradient_x(u, z)
+    probe = [1, m.num_nodes // 2, int(fv[0, 0]), int(fv[-1, 1])]
+    for n in probe:
+        for k in range(3):
+            h = 1e-6
+            Xp, Xm = m.coords.copy(), m.coords.copy()
+            Xp[n, k] += h
+            Xm[n, k] -= h
+            mp = orc.Model(Xp, m.conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3, face_verts=fv, traction=t)
+            mm = orc.Model(Xm, m.conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3, face_verts=fv, traction=t)
+            fd = (mp.residual(u, z) - mm.residual(u, z)) / (2 * h)
+            col = np.zeros(3 * m.num_nodes)
+            for s in range(rm[n], rm[n + 1]):
+                col[3 * ci[s]:3 * ci[s] + 3] = G[9 * s + 3 * k:9 * s + 3 * k + 3]
+            np.testing.assert_allclose(col, fd, atol=1e-6 * np.abs(fd).max() + 1e-12)
+            fe_ = (mp.energy(u, z, False, False)[0] - mm.energy(u, z, False, False)[0]) / (2 * h)
+            assert abs(fe_ - gE[3 * n + k]) <= 1e-6 * np.abs(gE).max(), (n, k, fe_, gE[3 * n + k])
 
 
 def test_constraints_and_pcg_vs_dense(small):
