@@ -72,7 +72,7 @@ struct MgLevel {
   int nn = 0;
   DevArray<int32_t> grid;          // (i, j, k) per node
   DevArray<int32_t> node_of_grid;  // node per grid point
-  DevArray<double> r, x, y, d, t, rr, w, zc, vals, dinv;
+  DevArray<double> r, x, y, d, t, rr, w, vals, dinv;  // w: nodal stiffness weight
   DevArray<double> pw;             // power-iteration vector
   double inv_theta = 0.0, c1 = 0.0, c2 = 0.0, lam = 0.0;
   ~MgLevel();
@@ -932,7 +932,6 @@ static int mg_level_init(MgLevel& L, const std::vector<int32_t>& grid) {
   PA_HIP(L.rr.alloc(nd));
   PA_HIP(L.pw.alloc(nd));
   PA_HIP(L.w.alloc(L.nn));
-  PA_HIP(L.zc.alloc(L.nn));
   PA_HIP(L.dinv.alloc(9 * (size_t)L.nn));
   return PA_OK;
 }
@@ -979,7 +978,9 @@ static int mg_build(pa_handle h) {
     L->h = hc;
     L->owned = true;
     if (int rc = pa_set_material(hc, h->E, h->nu)) return rc;
-    if (int rc = pa_set_penalty(hc, h->p, h->rmin)) return rc;
+    // coarse levels take the homogenised stiffness weight itself as their density
+    // (SIMP exponent 1, minimum 0: w = rho)
+    if (int rc = pa_set_penalty(hc, 1.0, 0.0)) return rc;
     // a coarse dof is constrained when the coincident node of the finer level is
     std::vector<uint8_t> pmask(P.nn, 0);
     for (int32_t dof : P.h->bc_dofs) pmask[dof / 3] |= (uint8_t)(1u << (dof % 3));
@@ -1213,8 +1214,7 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
     MgLevel& F = *h->mg[l - 1];
     MgLevel& L = *h->mg[l];
     PA_HIP(pa::launch_mg_wrestrict(F.w.p, L.w.p, L.grid.p, F.node_of_grid.p, F.dims, L.nn, s));
-    PA_HIP(pa::launch_simp_z(L.w.p, L.zc.p, L.nn, h->rmin, h->p, s));
-    if (int rc = run_chunk(L.h, pa::kModeJacobian, nullptr, L.zc.p, L.vals.p, nullptr)) return rc;
+    if (int rc = run_chunk(L.h, pa::kModeJacobian, nullptr, L.w.p, L.vals.p, nullptr)) return rc;
     if (int rc = pa_apply_constraints(L.h, L.vals.p, L.rr.p, 0.0)) return rc;
     pa::k_diag_inv<<<grid_for(L.nn), 256, 0, s>>>(L.vals.p, L.h->diag_slot.p, L.dinv.p, L.nn);
     PA_HIP(hipGetLastError());

@@ -498,40 +498,48 @@ __host__ __device__ inline JacLds jac_lds_layout(int max_rows, int max_blocks, i
 // Transposes (slot of the partner block from a setup table), diagonals and residual
 // are finished by 16-lane row groups with DPP reductions, then the chunk's CRS
 // segment is streamed out once.
-template <int BITS, bool WITH_R, int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
-void k_jac_chunk(ChunkArgs A) {
-  extern __shared__ double lds[];
-  const JacLds Lo = jac_lds_layout(A.max_rows, A.max_blocks, A.max_nodes);
-  char* base = reinterpret_cast<char*>(lds);
-  double* seg = reinterpret_cast<double*>(base + Lo.seg);
-  double4* nodes = reinterpret_cast<double4*>(base + Lo.nodes);
-  int* roff = reinterpret_cast<int*>(base + Lo.roff);
-  int* col = reinterpret_cast<int*>(base + Lo.col);
-  double* fl = reinterpret_cast<double*>(base + Lo.fl);
+// Rows, blocks and node table of one chunk.
+struct JacChunk {
+  int c, r0, nr, b0, nb, n0, nnode;
+};
+__device__ __forceinline__ JacChunk jac_chunk_info(const ChunkArgs& A, int c) {
+  JacChunk J;
+  J.c = c;
+  J.r0 = A.chunk_rows[c];
+  J.nr = A.chunk_rows[c + 1] - J.r0;
+  J.b0 = A.rowmap[J.r0];
+  J.nb = A.rowmap[J.r0 + J.nr] - J.b0;
+  J.n0 = A.node_offs[c];
+  J.nnode = A.node_offs[c + 1] - J.n0;
+  return J;
+}
 
-  const int c = xcd_chunk(blockIdx.x, A.nchunks);
-  const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1];
-  const int nr = r1 - r0;
-  const int b0 = A.rowmap[r0];
-  const int nb = A.rowmap[r1] - b0;
-  const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
-
+// Phase 1 (before a barrier): zero the chunk's CRS segment, stage row offsets, columns
+// and the node table; zero the load rows.
+template <int NT, bool WITH_R>
+__device__ __forceinline__ void jac_stage(const ChunkArgs& A, const JacChunk& J, double* seg, double4* nodes,
+                                          int* roff, int* col, double* fl) {
   {
     // seg starts 16-byte aligned and holds 9*nb doubles
     double2* s2 = reinterpret_cast<double2*>(seg);
-    const int n2 = (9 * nb + 1) / 2;
+    const int n2 = (9 * J.nb + 1) / 2;
     for (int k = threadIdx.x; k < n2; k += NT) s2[k] = make_double2(0.0, 0.0);
   }
-  for (int k = threadIdx.x; k <= nr; k += NT) roff[k] = A.rowmap[r0 + k] - b0;
-  for (int k = threadIdx.x; k < nb; k += NT) col[k] = A.colidx[b0 + k];
-  for (int k = threadIdx.x; k < nnode; k += NT) {
-    const size_t gid = (size_t)A.nodes[n0 + k];
+  for (int k = threadIdx.x; k <= J.nr; k += NT) roff[k] = A.rowmap[J.r0 + k] - J.b0;
+  for (int k = threadIdx.x; k < J.nb; k += NT) col[k] = A.colidx[J.b0 + k];
+  for (int k = threadIdx.x; k < J.nnode; k += NT) {
+    const size_t gid = (size_t)A.nodes[J.n0 + k];
     nodes[k] = make_double4(A.X[3 * gid], A.X[3 * gid + 1], A.X[3 * gid + 2], A.z[gid]);
   }
   if constexpr (WITH_R)
-    for (int k = threadIdx.x; k < nr * 3; k += NT) fl[k] = 0.0;
-  __syncthreads();
+    for (int k = threadIdx.x; k < J.nr * 3; k += NT) fl[k] = 0.0;
+}
+
+// Phase 2 (after the staging barrier): the chunk's traction loads, then the elements.
+template <int BITS, int NT, bool WITH_R>
+__device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk& J, double* seg,
+                                             const double4* nodes, const int* roff, double* fl) {
+  const int c = J.c, nr = J.nr;
   if constexpr (WITH_R) {
     if (A.sub_load)
       for (int k = A.load_offs[c] + threadIdx.x; k < A.load_offs[c + 1]; k += NT) {
@@ -541,7 +549,6 @@ void k_jac_chunk(ChunkArgs A) {
         fl[3 * lr + 2] = A.load_f[3 * k + 2];
       }
   }
-
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
   for (int k = l0 + threadIdx.x; k < l1; k += NT) {
     const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
@@ -602,8 +609,13 @@ void k_jac_chunk(ChunkArgs A) {
         for (int j = 0; j < 3; ++j) lds_add(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j]);
     }
   }
-  __syncthreads();
+}
 
+// Phase 3 (after the element barrier): row groups finish the rows.
+template <int NT, bool WITH_R>
+__device__ __forceinline__ void jac_rows(const ChunkArgs& A, const JacChunk& J, double* seg, const int* roff,
+                                         const int* col, const double* fl) {
+  const int r0 = J.r0, nr = J.nr, b0 = J.b0;
   // Row groups of 16 lanes finish one row each: transposed blocks, diagonal, residual.
   constexpr int kG = 16;
   const int lane = threadIdx.x % kG;
@@ -669,7 +681,11 @@ void k_jac_chunk(ChunkArgs A) {
       }
     }
   }
-  __syncthreads();
+}
+
+// Phase 4 (after the row barrier): stream the segment out.
+template <int NT>
+__device__ __forceinline__ void jac_stream(const ChunkArgs& A, int b0, int nb, const double* seg) {
   // Stream the segment out, 16 bytes per store where the destination allows, with
   // non-temporal stores: nothing in this kernel reads the values again, and bypassing
   // the caches measured 12 % faster on the 128x64x64 workload.
@@ -684,6 +700,27 @@ void k_jac_chunk(ChunkArgs A) {
     __builtin_nontemporal_store(v2d{seg[e], seg[e + 1]}, reinterpret_cast<v2d*>(dst + e));
   }
   if (((n - head) & 1) && threadIdx.x == 0) __builtin_nontemporal_store(seg[n - 1], dst + n - 1);
+}
+
+template <int BITS, bool WITH_R, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT >= 512 ? 4 : 1)))
+void k_jac_chunk(ChunkArgs A) {
+  extern __shared__ double lds[];
+  const JacLds Lo = jac_lds_layout(A.max_rows, A.max_blocks, A.max_nodes);
+  char* base = reinterpret_cast<char*>(lds);
+  double* seg = reinterpret_cast<double*>(base + Lo.seg);
+  double4* nodes = reinterpret_cast<double4*>(base + Lo.nodes);
+  int* roff = reinterpret_cast<int*>(base + Lo.roff);
+  int* col = reinterpret_cast<int*>(base + Lo.col);
+  double* fl = reinterpret_cast<double*>(base + Lo.fl);
+  const JacChunk J = jac_chunk_info(A, xcd_chunk(blockIdx.x, A.nchunks));
+  jac_stage<NT, WITH_R>(A, J, seg, nodes, roff, col, fl);
+  __syncthreads();
+  jac_elements<BITS, NT, WITH_R>(A, J, seg, nodes, roff, fl);
+  __syncthreads();
+  jac_rows<NT, WITH_R>(A, J, seg, roff, col, fl);
+  __syncthreads();
+  jac_stream<NT>(A, J.b0, J.nb, seg);
 }
 
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {

@@ -127,12 +127,6 @@ __global__ __launch_bounds__(kT) void k_simp_w(const double* z, double* w, int n
   for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) w[i] = rmin + (1.0 - rmin) * pow(z[i], p);
 }
 
-__global__ __launch_bounds__(kT) void k_simp_z(const double* w, double* z, int n, double rmin, double p) {
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) {
-    const double t = (w[i] - rmin) / (1.0 - rmin);
-    z[i] = t > 0.0 ? pow(fmin(t, 1.0), 1.0 / p) : 0.0;
-  }
-}
 
 __global__ __launch_bounds__(kT) void k_cheb_first(double* __restrict__ x, double* __restrict__ d,
                                                    const double* __restrict__ r, const double* __restrict__ dinv,
@@ -322,10 +316,6 @@ hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgri
 }
 hipError_t launch_simp_w(const double* z, double* w, int n, double rmin, double p, hipStream_t s) {
   k_simp_w<<<blocks_of(n), kT, 0, s>>>(z, w, n, rmin, p);
-  return hipGetLastError();
-}
-hipError_t launch_simp_z(const double* w, double* z, int n, double rmin, double p, hipStream_t s) {
-  k_simp_z<<<blocks_of(n), kT, 0, s>>>(w, z, n, rmin, p);
   return hipGetLastError();
 }
 hipError_t launch_cheb_first(double* x, double* d, const double* r, const double* dinv, double inv_theta, int nn,

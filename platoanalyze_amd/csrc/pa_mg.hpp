@@ -27,9 +27,9 @@ hipError_t launch_mg_prolong_add(const double* ec, double* xf, const int32_t* fg
 // SIMP stiffness weight): wc[c] = sum w_i wf[i] / sum w_i.
 hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgrid, const int32_t* fnode_of_grid,
                                BoxDims fd, int ncn, hipStream_t s);
-// w = rmin + (1 - rmin) z^p and its inverse z = ((w - rmin) / (1 - rmin))^(1/p)
+// nodal stiffness weight w = rmin + (1 - rmin) z^p (coarse levels use w as their density
+// under the identity penalty)
 hipError_t launch_simp_w(const double* z, double* w, int n, double rmin, double p, hipStream_t s);
-hipError_t launch_simp_z(const double* w, double* z, int n, double rmin, double p, hipStream_t s);
 
 // Chebyshev smoothing of K x = r with block-Jacobi D^-1, from x = 0:
 //   first: d = D^-1 r / theta; x = d
