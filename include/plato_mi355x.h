@@ -80,6 +80,16 @@ int pa_set_material(pa_handle h, double youngs_modulus, double poissons_ratio);
 /* MSIMP penalty: "Exponent" p, "Minimum Value" rho_min (src/Simp.hpp:44-50,130-134):
  * w = rho_min + (1-rho_min)*pow(mean(z_e), p).  Defaults 3 and 1e-9 as the reference. */
 int pa_set_penalty(pa_handle h, double exponent, double minimum_value);
+/* Penalty model of the stiffness weight w(rho) applied to the stress (ApplyWeighting,
+ * src/ApplyWeighting.hpp:49-65); "Penalty Function" Type of the reference's input:
+ *   0 SIMP       w = rmin + (1 - rmin) rho^p                      (src/Simp.hpp:130-134)
+ *   1 RAMP       w = rmin + (1 - rmin) rho / (1 + p (1 - rho))    (src/Ramp.hpp)
+ *   2 Heaviside  w = rmin + (1 - rmin) (0.5 (1 + sin(pi rho / (2 L))))^p on |rho| < L,
+ *                rmin below -L, 1 above L                         (src/Heaviside.hpp)
+ *   3 none       w = 1                                            (src/NoPenalty.hpp)
+ * exponent = "Exponent", minimum_value = "Minimum Value", length = "Regularization
+ * Length" (Heaviside only).  pa_set_penalty(h, p, rmin) is model 0. */
+int pa_set_penalty_model(pa_handle h, int model, double exponent, double minimum_value, double length);
 
 /* Essential BC on one dof of a node set (src/EssentialBCs.hpp:185-218), value scaled like
  * the reference's step scale in applyBlockConstraints. */
@@ -131,6 +141,12 @@ int pa_vf_gradient_x(pa_handle h, const double* u, const double* z, double* valu
 int pa_sf_value(pa_handle h, const double* u, const double* z, double* value);
 int pa_sf_gradient_u(pa_handle h, const double* u, const double* z, double* g);
 int pa_sf_gradient_z(pa_handle h, const double* u, const double* z, double* g, double* value);
+/* Geometric "Volume" criterion (src/geometric/Volume.hpp:49-96) with its own penalty
+ * function (model as pa_set_penalty_model): V = sum_e vol_e w(mean z_e); dV/dz (N) and
+ * dV/dX (3N) as GeometryScalarFunction gradient_z / gradient_x
+ * (src/geometric/GeometryScalarFunction.cpp).  value (host), dvdz, dvdx may be NULL. */
+int pa_volume(pa_handle h, const double* z, int model, double exponent, double minimum_value, double length,
+              double* value, double* dvdz, double* dvdx);
 /* gradient_x = partial dC/dX at fixed u and z (3N)
  *                                       (PhysicsScalarFunction.hpp:293-355) */
 int pa_sf_gradient_x(pa_handle h, const double* u, const double* z, double* g);

@@ -27,6 +27,10 @@
 //                                      differences of orc_residual in the coordinates
 //   orc_energy_gradient_x ............ DerivativeTests.cpp:665-693 (81 values); central
 //                                      differences of orc_energy in the coordinates
+//   penalty models (RAMP, Heaviside, none) and orc_volume ... parity unpinned by the
+//                                      reference (no elliptic goldens): closed forms
+//                                      (box volume, limits of each model) and central
+//                                      differences in the densities and coordinates
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -150,7 +154,13 @@ struct ValueOf<double> {
 struct Params {
   double c[6][6];
   double p, rmin;
+  int model;   // 0 SIMP, 1 RAMP, 2 Heaviside, 3 none
+  double len;  // Heaviside regularization length
 };
+
+// Penalty model for the following evaluations (set by orc_set_penalty_model)
+int g_penalty_model = 0;
+double g_penalty_len = 1.0;
 
 void make_params(double E, double nu, double p, double rmin, Params& P) {
   // IsotropicLinearElasticMaterial<3> (src/IsotropicLinearElasticMaterial.cpp:87-127)
@@ -170,6 +180,8 @@ void make_params(double E, double nu, double p, double rmin, Params& P) {
   P.c[5][5] = 1.0 / 2.0 * coef * (1.0 - 2.0 * nu);
   P.p = p;
   P.rmin = rmin;
+  P.model = g_penalty_model;
+  P.len = g_penalty_len;
 }
 
 // ComputeGradientWorkset<3> (src/ImplicitFunctors.hpp:165-217) with the cubature
@@ -224,12 +236,53 @@ void stress(const Params& P, const S* e, S* s) {
   }
 }
 
-// ApplyWeighting<3,6,MSIMP> (src/ApplyWeighting.hpp:49-65) + MSIMP (src/Simp.hpp:130-134)
+// Penalty functions w(rho) and dw/drho: RAMP (src/Ramp.hpp), Heaviside
+// (src/Heaviside.hpp), NoPenalty (src/NoPenalty.hpp); SIMP below through fad_pow.
+double penalty_value(const Params& P, double rho, double* dw) {
+  switch (P.model) {
+    case 1: {
+      const double d = 1.0 + P.p * (1.0 - rho);
+      *dw = (1.0 - P.rmin) * (1.0 + P.p) / (d * d);
+      return P.rmin + (1.0 - P.rmin) * rho / d;
+    }
+    case 2: {
+      if (rho <= -P.len) {
+        *dw = 0.0;
+        return P.rmin;
+      }
+      if (rho >= P.len) {
+        *dw = 0.0;
+        return 1.0;
+      }
+      const double a = M_PI / (2.0 * P.len), h = 0.5 * (1.0 + std::sin(a * rho));
+      *dw = (1.0 - P.rmin) * P.p * std::pow(h, P.p - 1.0) * 0.5 * a * std::cos(a * rho);
+      return P.rmin + (1.0 - P.rmin) * std::pow(h, P.p);
+    }
+    default:
+      *dw = 0.0;
+      return 1.0;
+  }
+}
+inline double penalty_of(const Params& P, double rho) {
+  double dw;
+  return penalty_value(P, rho, &dw);
+}
+template <int N>
+inline Fad<N> penalty_of(const Params& P, const Fad<N>& rho) {
+  double dw;
+  Fad<N> r(penalty_value(P, rho.v, &dw));
+  for (int i = 0; i < N; ++i) r.dx[i] = dw * rho.dx[i];
+  return r;
+}
+
+// ApplyWeighting<3,6,Penalty> (src/ApplyWeighting.hpp:49-65) with MSIMP
+// (src/Simp.hpp:130-134) or the model chosen by orc_set_penalty_model
 template <typename Z>
 Z msimp_weight(const Params& P, const Z* z /*4*/) {
   Z rho(0.0);
   for (int n = 0; n < 4; ++n) rho += z[n];
   rho = rho / 4.0;
+  if (P.model != 0) return penalty_of(P, rho);
   return P.rmin + (1.0 - P.rmin) * fad_pow(rho, P.p);
 }
 
@@ -339,6 +392,11 @@ void traction(const Mesh& m, int nfaces, const int* fv, const double* t, double*
 }  // namespace
 
 extern "C" {
+
+void orc_set_penalty_model(int model, double len) {
+  g_penalty_model = model;
+  g_penalty_len = len;
+}
 
 int orc_num_threads(void) { return omp_get_max_threads(); }
 void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
@@ -957,6 +1015,54 @@ void orc_energy_gradient_x(int nn, int ne, const double* X, const int* conn, con
     F val = inc * vol * 0.5;
     for (int k = 0; k < 12; ++k) atomic_add(&gx[3 * (size_t)conn[4 * e + k / 3] + k % 3], val.dx[k]);
   }
+}
+
+// Plato::Geometric::Volume (src/geometric/Volume.hpp:49-96) through
+// GeometryScalarFunction: V = sum_e (|det J|/6) w(mean z_e) with the penalty
+// (model, p, rmin, len); dV/dz by SFad<4> on the controls, dV/dX by SFad<12> on the
+// cell coordinates.  dvdz (N) and dvdx (3N) may be NULL.
+double orc_volume(int nn, int ne, const double* X, const int* conn, const double* z, int model, double p,
+                  double rmin, double len, double* dvdz, double* dvdx) {
+  Mesh m{nn, ne, X, conn};
+  Params P;
+  make_params(1.0, 0.3, p, rmin, P);
+  P.model = model;
+  P.len = len;
+  auto cfgw = workset_config(m);
+  double total = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : total)
+  for (int e = 0; e < ne; ++e) {
+    double g[12], v, zc[4];
+    gradient_workset(&cfgw[12 * (size_t)e], g, v);
+    for (int n = 0; n < 4; ++n) zc[n] = z[conn[4 * e + n]];
+    total += v * msimp_weight(P, zc);
+  }
+  if (dvdz) {
+    std::memset(dvdz, 0, sizeof(double) * (size_t)nn);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+      double g[12], v;
+      gradient_workset(&cfgw[12 * (size_t)e], g, v);
+      Fad<4> zc[4];
+      for (int n = 0; n < 4; ++n) zc[n] = Fad<4>(z[conn[4 * e + n]], n);
+      Fad<4> val = msimp_weight(P, zc) * v;
+      for (int n = 0; n < 4; ++n) atomic_add(&dvdz[conn[4 * e + n]], val.dx[n]);
+    }
+  }
+  if (dvdx) {
+    std::memset(dvdx, 0, sizeof(double) * 3 * (size_t)nn);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+      Fad<12> cfg[12], g[12], v;
+      double zc[4];
+      for (int k = 0; k < 12; ++k) cfg[k] = Fad<12>(cfgw[12 * (size_t)e + k], k);
+      for (int n = 0; n < 4; ++n) zc[n] = z[conn[4 * e + n]];
+      gradient_workset_t(cfg, g, v);
+      Fad<12> val = v * msimp_weight(P, zc);
+      for (int k = 0; k < 12; ++k) atomic_add(&dvdx[3 * (size_t)conn[4 * e + k / 3] + k % 3], val.dx[k]);
+    }
+  }
+  return total;
 }
 
 }  // extern "C"

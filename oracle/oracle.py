@@ -22,6 +22,7 @@ def lib():
         _lib = C.CDLL(str(LIB))
         _lib.orc_stress_pnorm.restype = C.c_double
         _lib.orc_pnorm.restype = C.c_double
+        _lib.orc_volume.restype = C.c_double
     return _lib
 
 
@@ -59,19 +60,29 @@ def graph(conn, nn):
 class Model:
     """Mesh + material + penalty (+ traction faces) for the oracle calls."""
 
-    def __init__(self, coords, conn, E=1.0, nu=0.3, p=3.0, rmin=1e-9, face_verts=None, traction=None):
+    def __init__(self, coords, conn, E=1.0, nu=0.3, p=3.0, rmin=1e-9, face_verts=None, traction=None,
+                 penalty_model=0, length=1.0):
         self.X = _f64(coords).reshape(-1, 3)
         self.conn = _i32(conn).reshape(-1, 4)
         self.nn, self.ne = self.X.shape[0], self.conn.shape[0]
         self.E, self.nu, self.p, self.rmin = float(E), float(nu), float(p), float(rmin)
         self.fv = None if face_verts is None else _i32(face_verts).reshape(-1, 3)
         self.t = None if traction is None else _f64(traction)
+        self.model, self.len = int(penalty_model), float(length)
 
     def _m(self):
         return (self.nn, self.ne, _p(self.X), _p(self.conn))
 
     def _mat(self):
+        lib().orc_set_penalty_model(self.model, C.c_double(self.len))  # penalty of this model's calls
         return (C.c_double(self.E), C.c_double(self.nu), C.c_double(self.p), C.c_double(self.rmin))
+
+    def volume(self, z, model=3, p=1.0, rmin=0.0, length=1.0):
+        """Geometric volume sum_e vol_e w(mean z_e) and its gradients (dV/dz, dV/dX)."""
+        gz, gx = np.empty(self.nn), np.empty(3 * self.nn)
+        v = lib().orc_volume(*self._m(), _p(_f64(z)), int(model), C.c_double(p), C.c_double(rmin),
+                             C.c_double(length), _p(gz), _p(gx))
+        return float(v), gz, gx
 
     def residual(self, u, z):
         R = np.empty(3 * self.nn)

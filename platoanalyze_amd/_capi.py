@@ -77,6 +77,8 @@ _SIGS = {
                                       C.POINTER(SolverStats)]),
     "pa_smooth": (_I, [_P, _P, _P, _I, _D]),
     "pa_set_box": (_I, [_P, _I, _I, _I]),
+    "pa_set_penalty_model": (_I, [_P, _I, _D, _D, _D]),
+    "pa_volume": (_I, [_P, _P, _I, _D, _D, _D, _PD, _P, _P]),
     "pa_unet_last_error": (C.c_char_p, []),
     "pa_unet_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, C.c_ulonglong]),
     "pa_unet_destroy": (_I, [_P]),

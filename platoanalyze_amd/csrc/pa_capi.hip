@@ -84,6 +84,8 @@ struct pa_problem {
   std::vector<int32_t> conn;
   double E = 1.0, nu = 0.3;
   double p = 3.0, rmin = 1e-9;
+  int pmodel = 0;       // pa::PenaltyModel
+  double plen = 1.0;    // Heaviside regularization length
   std::vector<int32_t> bc_dofs;
   std::vector<double> bc_vals;
   std::vector<double> f;  // dense load, filled by pa_add_traction
@@ -156,6 +158,8 @@ struct pa_problem {
     m.p = p;
     m.rmin = rmin;
     m.pint = (p >= 1.0 && p <= 8.0 && p == std::floor(p)) ? (int)p : 0;
+    m.model = pmodel;
+    m.len = plen;
     return m;
   }
   pa::ChunkArgs chunk_args() const {
@@ -286,10 +290,19 @@ int pa_set_material(pa_handle h, double E, double nu) {
 }
 
 int pa_set_penalty(pa_handle h, double exponent, double minimum_value) {
+  return pa_set_penalty_model(h, pa::kPenaltySimp, exponent, minimum_value, 1.0);
+}
+
+int pa_set_penalty_model(pa_handle h, int model, double exponent, double minimum_value, double length) {
   if (!h) return fail(PA_ERR_ARG, "null handle");
-  if (!std::isfinite(exponent) || !std::isfinite(minimum_value)) return fail(PA_ERR_ARG, "non-finite penalty");
+  if (!std::isfinite(exponent) || !std::isfinite(minimum_value) || !std::isfinite(length))
+    return fail(PA_ERR_ARG, "non-finite penalty");
+  if (model < pa::kPenaltySimp || model > pa::kPenaltyNone) return fail(PA_ERR_ARG, "unknown penalty model");
+  if (model == pa::kPenaltyHeaviside && !(length > 0.0)) return fail(PA_ERR_ARG, "Heaviside length must be > 0");
+  h->pmodel = model;
   h->p = exponent;
   h->rmin = minimum_value;
+  h->plen = length;
   return PA_OK;
 }
 
@@ -573,6 +586,34 @@ int pa_sf_gradient_x(pa_handle h, const double* u, const double* z, double* g) {
   if (int rc = ready(h)) return rc;
   if (!u || !z || !g) return fail(PA_ERR_ARG, "pa_sf_gradient_x: null pointer");
   return run_chunk(h, pa::kModeEnergyGradX, u, z, nullptr, g);
+}
+
+static int energy_value(pa_handle h, double* value);
+static int ensure_solver_ws(pa_handle h);
+
+// Plato::Geometric::Volume (src/geometric/Volume.hpp:49-96) with its own penalty
+// function, through GeometryScalarFunction value / gradient_z / gradient_x
+// (src/geometric/GeometryScalarFunction.cpp).
+int pa_volume(pa_handle h, const double* z, int model, double exponent, double minimum_value, double length,
+              double* value, double* dvdz, double* dvdx) {
+  if (int rc = ready(h)) return rc;
+  if (!z) return fail(PA_ERR_ARG, "pa_volume: null density");
+  if (model < pa::kPenaltySimp || model > pa::kPenaltyNone) return fail(PA_ERR_ARG, "unknown penalty model");
+  if (model == pa::kPenaltyHeaviside && !(length > 0.0)) return fail(PA_ERR_ARG, "Heaviside length must be > 0");
+  if (int rc = ensure_solver_ws(h)) return rc;
+  pa::ChunkArgs a = h->chunk_args();
+  a.u = h->w_r.p;  // staged but not used by the volume mode
+  a.z = z;
+  a.m.model = model;
+  a.m.p = exponent;
+  a.m.rmin = minimum_value;
+  a.m.len = length;
+  a.m.pint = (exponent >= 1.0 && exponent <= 8.0 && exponent == std::floor(exponent)) ? (int)exponent : 0;
+  a.out_vec = dvdx ? dvdx : h->w_p.p;   // w_p holds 3N doubles
+  a.out_vec2 = dvdz ? dvdz : h->w_z.p;  // w_z holds >= N doubles
+  PA_HIP(pa::launch_chunk(h->code_bits, pa::kModeVolume, a, h->stream));
+  if (value) return energy_value(h, value);
+  return PA_OK;
 }
 
 // --------------------------------------------------------------- scalar function
@@ -1209,7 +1250,7 @@ static int spd_inverse(std::vector<double>& A, std::vector<double>& U, int n) {
 static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixed) {
   hipStream_t s = h->stream;
   const size_t nl = h->mg.size();
-  PA_HIP(pa::launch_simp_w(z, h->mg[0]->w.p, h->nn, h->rmin, h->p, s));
+  PA_HIP(pa::launch_simp_w(z, h->mg[0]->w.p, h->nn, h->material(), s));
   for (size_t l = 1; l < nl; ++l) {
     MgLevel& F = *h->mg[l - 1];
     MgLevel& L = *h->mg[l];

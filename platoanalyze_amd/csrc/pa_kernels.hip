@@ -85,16 +85,6 @@ __device__ __forceinline__ int sel4i(int a, int x0, int x1, int x2, int x3) {
   return a == 0 ? x0 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
 }
 
-// MSIMP weight pow(rho, p): small integer exponents take a multiply chain (uniform
-// branch on a kernel argument); anything else uses pow().
-__device__ __forceinline__ double simp_pow(double rho, double p, int pint) {
-  if (pint > 0) {
-    double r = rho;
-    for (int k = 1; k < pint; ++k) r *= rho;
-    return r;
-  }
-  return pow(rho, p);
-}
 
 // Vector-valued element kernels (residual, dR/dz, energy and its gradient) over the
 // same chunks: node coordinates, density and state staged in LDS, elements streamed
@@ -107,11 +97,11 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   constexpr bool kGradZ = MODE == kModeEnergyGradZ;
   constexpr bool kPnV = MODE == kModePNorm, kPnG = MODE == kModePNormGrad;
   constexpr bool kAdj = MODE == kModeAdjZ;
-  constexpr bool kGx = MODE == kModeGradX, kEGx = MODE == kModeEnergyGradX;
-  constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV;
+  constexpr bool kGx = MODE == kModeGradX, kEGx = MODE == kModeEnergyGradX, kVol = MODE == kModeVolume;
+  constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV || kVol;
   constexpr int kBlkW = kGzT ? 3 : (kGx ? 9 : 0);                      // doubles per block in LDS
-  constexpr int kRowW = (kRes || kPnG || kEGx) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row in LDS
-  constexpr int kRow2 = kPnG ? 1 : 0;                                  // second per-row output
+  constexpr int kRowW = (kRes || kPnG || kEGx || kVol) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row
+  constexpr int kRow2 = (kPnG || kVol) ? 1 : 0;                        // second per-row output
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
   const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1];
@@ -167,7 +157,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     }
     tet_gradients(x, g, vol);
     rho = rho / 4.0;
-    const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
+    const double w = penalty_w(A.m, rho);
     double eps[6], s[6];
     tet_strain(u, g, eps);
     iso_stress(A.m, eps, s);
@@ -183,7 +173,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
         double sr = 0.0;
 #pragma unroll
         for (int q = 0; q < 6; ++q) sr += s[q] * eps[q];
-        const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
+        const double dw = penalty_dw(A.m, rho);
         const double gz = 0.5 * vol * sr * dw * 0.25;
 #pragma unroll
         for (int a = 0; a < 4; ++a)
@@ -202,7 +192,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
 #pragma unroll
         for (int q = 0; q < 6; ++q) t[q] = w * s[q];  // d|sigma_w|/du = w B^T C0 sigma_w / m
         cu = mval > 0.0 ? w / mval : 0.0;
-        cz = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1) / w;
+        cz = penalty_dw(A.m, rho) / w;
       } else {
         const double d01 = s[0] - s[1], d12 = s[1] - s[2], d20 = s[2] - s[0];
         const double j2 = 0.5 * (d01 * d01 + d12 * d12 + d20 * d20) + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
@@ -252,7 +242,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
       double dot = 0.0;
 #pragma unroll
       for (int q = 0; q < 6; ++q) dot += el[q] * s[q];
-      const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1);
+      const double dw = penalty_dw(A.m, rho);
       const double v = 0.25 * dw * vol * dot;
 #pragma unroll
       for (int a = 0; a < 4; ++a)
@@ -290,7 +280,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
         cw[2] = t.z;
         cw[3] = t.w;
       }
-      const double dw = (1.0 - A.m.rmin) * A.m.p * simp_pow(rho, A.m.p - 1.0, A.m.pint - 1) * 0.25;
+      const double dw = penalty_dw(A.m, rho) * 0.25;
       double rb[4][3];
 #pragma unroll
       for (int b = 0; b < 4; ++b) tet_divergence(s, g[b], vol, rb[b]);
@@ -305,6 +295,20 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
 #pragma unroll
           for (int i = 0; i < 3; ++i) lds_add(&seg[3 * sl + i], dw * rb[b][i]);
         }
+      }
+    }
+    if constexpr (kVol) {
+      // Geometric volume V = sum_e vol_e w(rho_e) with the criterion's own penalty (in
+      // A.m): dV/dz_a = vol w'(rho) / 4, dV/dX_a = w vol g_a (dvol/dX_a = vol g_a)
+      const double dw = penalty_dw(A.m, rho);
+      if (ln[0] < nr) energy += vol * w;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        if (ln[a] >= nr) continue;
+        double* dst = &res[3 * ln[a]];
+#pragma unroll
+        for (int k2 = 0; k2 < 3; ++k2) lds_add(dst + k2, w * vol * g[a][k2]);
+        lds_add(&res2[ln[a]], 0.25 * vol * dw);
       }
     }
     if constexpr (kGx || kEGx) {
@@ -578,7 +582,7 @@ __device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk&
     }
     tet_gradients(x, g, vol);
     rho = rho / 4.0;
-    const double w = A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint);
+    const double w = penalty_w(A.m, rho);
     const double vl = vol * w * A.m.c01, vm = vol * w * A.m.c33;
     bool own[4];
     int ro[4];
@@ -726,9 +730,10 @@ void k_jac_chunk(ChunkArgs A) {
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
   if (mode == kModeJacobian || mode == kModeJacobianResidual) return jac_lds_layout(max_rows, max_blocks, max_nodes).total;
   const int blkw = mode == kModeGradZT ? 3 : (mode == kModeGradX ? 9 : 0);
-  const int roww = (mode == kModeResidual || mode == kModePNormGrad || mode == kModeEnergyGradX) ? 3
-                   : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
-  const int row2 = mode == kModePNormGrad ? 1 : 0;
+  const int roww = (mode == kModeResidual || mode == kModePNormGrad || mode == kModeEnergyGradX || mode == kModeVolume)
+                       ? 3
+                       : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
+  const int row2 = (mode == kModePNormGrad || mode == kModeVolume) ? 1 : 0;
   const int nodew = mode == kModeAdjZ ? 10 : 7;
   size_t bytes = sizeof(double) * (nodew * (size_t)max_nodes + (size_t)max_blocks * blkw +
                                    (size_t)max_rows * (roww + row2));
@@ -757,6 +762,7 @@ static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
     case kModeAdjZ: k_chunk<BITS, kModeAdjZ, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeGradX: k_chunk<BITS, kModeGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeEnergyGradX: k_chunk<BITS, kModeEnergyGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModeVolume: k_chunk<BITS, kModeVolume, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

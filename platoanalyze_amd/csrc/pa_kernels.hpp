@@ -19,11 +19,59 @@ namespace pa {
 constexpr int kThreads = 256;
 constexpr int kNumXcd = 8;
 
+// Penalty models of the stiffness weight w(rho) (ApplyWeighting, src/ApplyWeighting.hpp):
+// MSIMP (src/Simp.hpp:130-134), RAMP (src/Ramp.hpp), Heaviside (src/Heaviside.hpp),
+// NoPenalty (src/NoPenalty.hpp).
+enum PenaltyModel { kPenaltySimp = 0, kPenaltyRamp = 1, kPenaltyHeaviside = 2, kPenaltyNone = 3 };
+
 struct Material {
   double c00, c01, c33;  // Voigt stiffness: c(1-nu), c*nu, c(1-2nu)/2
-  double p, rmin;        // MSIMP exponent and minimum value
+  double p, rmin;        // penalty exponent and minimum value
   int pint;              // p as a small positive integer (1..8), else 0
+  int model;             // PenaltyModel
+  double len;            // Heaviside regularization length
 };
+
+// integer exponents by multiplication (uniform branch on a kernel argument), else pow()
+__device__ __forceinline__ double simp_pow(double rho, double p, int pint) {
+  if (pint > 0) {
+    double r = rho;
+    for (int k = 1; k < pint; ++k) r *= rho;
+    return r;
+  }
+  return pow(rho, p);
+}
+
+// w(rho) and dw/drho of the material's penalty model
+__device__ __forceinline__ double penalty_w(const Material& m, double rho) {
+  switch (m.model) {
+    case kPenaltyRamp: return m.rmin + (1.0 - m.rmin) * rho / (1.0 + m.p * (1.0 - rho));
+    case kPenaltyHeaviside: {
+      if (rho <= -m.len) return m.rmin;
+      if (rho >= m.len) return 1.0;
+      const double h = 0.5 * (1.0 + sin(M_PI * rho / (2.0 * m.len)));
+      return m.rmin + (1.0 - m.rmin) * pow(h, m.p);
+    }
+    case kPenaltyNone: return 1.0;
+    default: return m.rmin + (1.0 - m.rmin) * simp_pow(rho, m.p, m.pint);
+  }
+}
+__device__ __forceinline__ double penalty_dw(const Material& m, double rho) {
+  switch (m.model) {
+    case kPenaltyRamp: {
+      const double d = 1.0 + m.p * (1.0 - rho);
+      return (1.0 - m.rmin) * (1.0 + m.p) / (d * d);
+    }
+    case kPenaltyHeaviside: {
+      if (rho <= -m.len || rho >= m.len) return 0.0;
+      const double a = M_PI / (2.0 * m.len);
+      const double h = 0.5 * (1.0 + sin(a * rho));
+      return (1.0 - m.rmin) * m.p * pow(h, m.p - 1.0) * 0.5 * a * cos(a * rho);
+    }
+    case kPenaltyNone: return 0.0;
+    default: return (1.0 - m.rmin) * m.p * simp_pow(rho, m.p - 1.0, m.pint - 1);
+  }
+}
 
 // Arguments of the chunked element kernels (one work-group per row chunk).
 struct ChunkArgs {
@@ -66,7 +114,7 @@ struct ChunkArgs {
 
 enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
                  kModeEnergy = 4, kModeEnergyGradZ = 5, kModePNorm = 6, kModePNormGrad = 7, kModeAdjZ = 8,
-                 kModeGradX = 9, kModeEnergyGradX = 10, kNumChunkModes = 11 };
+                 kModeGradX = 9, kModeEnergyGradX = 10, kModeVolume = 11, kNumChunkModes = 12 };
 
 
 // launch helpers (defined in pa_kernels.hip)

@@ -109,7 +109,8 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
     }
     gradients_t(x, g, vol);
     rho *= 0.25f;
-    const float vw = vol * (rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint));
+    const float vw = vol * (A.m.model == kPenaltySimp ? rmin + (1.0f - rmin) * powi_f(rho, pp, A.m.pint)
+                                                      : (float)penalty_w(A.m, (double)rho));
     float e[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
@@ -174,14 +175,7 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
     }
     gradients_t(x, g, vol);
     rho = rho / 4.0;
-    double pw;
-    if (A.m.pint > 0) {
-      pw = rho;
-      for (int q = 1; q < A.m.pint; ++q) pw *= rho;
-    } else {
-      pw = pow(rho, A.m.p);
-    }
-    const double vw = vol * (A.m.rmin + (1.0 - A.m.rmin) * pw);
+    const double vw = vol * penalty_w(A.m, rho);
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
       if (ln[a] >= nr) continue;

@@ -3,6 +3,7 @@
 // (2I, 2J, 2K).  Trilinear prolongation: a fine coordinate 2I contributes weight 1 to
 // coarse I, 2I +- 1 weight 1/2 to each neighbour; restriction is its transpose.
 #include "pa_mg.hpp"
+#include "pa_kernels.hpp"
 
 #include <algorithm>
 
@@ -123,10 +124,9 @@ __global__ __launch_bounds__(kT) void k_mg_wrestrict(const double* __restrict__ 
   }
 }
 
-__global__ __launch_bounds__(kT) void k_simp_w(const double* z, double* w, int n, double rmin, double p) {
-  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) w[i] = rmin + (1.0 - rmin) * pow(z[i], p);
+__global__ __launch_bounds__(kT) void k_simp_w(const double* z, double* w, int n, Material m) {
+  for (int i = blockIdx.x * kT + threadIdx.x; i < n; i += gridDim.x * kT) w[i] = penalty_w(m, z[i]);
 }
-
 
 __global__ __launch_bounds__(kT) void k_cheb_first(double* __restrict__ x, double* __restrict__ d,
                                                    const double* __restrict__ r, const double* __restrict__ dinv,
@@ -314,8 +314,8 @@ hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgri
   k_mg_wrestrict<<<blocks_of(ncn), kT, 0, s>>>(wf, wc, cgrid, fnode_of_grid, fd, ncn);
   return hipGetLastError();
 }
-hipError_t launch_simp_w(const double* z, double* w, int n, double rmin, double p, hipStream_t s) {
-  k_simp_w<<<blocks_of(n), kT, 0, s>>>(z, w, n, rmin, p);
+hipError_t launch_simp_w(const double* z, double* w, int n, const Material& m, hipStream_t s) {
+  k_simp_w<<<blocks_of(n), kT, 0, s>>>(z, w, n, m);
   return hipGetLastError();
 }
 hipError_t launch_cheb_first(double* x, double* d, const double* r, const double* dinv, double inv_theta, int nn,

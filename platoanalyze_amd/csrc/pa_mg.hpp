@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "pa_kernels.hpp"
+
 namespace pa {
 
 struct BoxDims {
@@ -27,9 +29,9 @@ hipError_t launch_mg_prolong_add(const double* ec, double* xf, const int32_t* fg
 // SIMP stiffness weight): wc[c] = sum w_i wf[i] / sum w_i.
 hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgrid, const int32_t* fnode_of_grid,
                                BoxDims fd, int ncn, hipStream_t s);
-// nodal stiffness weight w = rmin + (1 - rmin) z^p (coarse levels use w as their density
-// under the identity penalty)
-hipError_t launch_simp_w(const double* z, double* w, int n, double rmin, double p, hipStream_t s);
+// nodal stiffness weight w = w(z) of the material's penalty model (coarse levels use w
+// as their density under the identity penalty)
+hipError_t launch_simp_w(const double* z, double* w, int n, const Material& m, hipStream_t s);
 
 // Chebyshev smoothing of K x = r with block-Jacobi D^-1, from x = 0:
 //   first: d = D^-1 r / theta; x = d

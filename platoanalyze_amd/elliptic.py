@@ -31,6 +31,17 @@ def _hptr(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
 
 
+PENALTY_MODELS = {"SIMP": 0, "RAMP": 1, "Heaviside": 2, "NoPenalty": 3}
+
+
+def penalty_model_id(name: str) -> int:
+    """Index of a "Penalty Function" Type (src/Simp.hpp, src/Ramp.hpp, src/Heaviside.hpp,
+    src/NoPenalty.hpp)."""
+    if name not in PENALTY_MODELS:
+        raise ValueError(f"penalty model must be one of {sorted(PENALTY_MODELS)}")
+    return PENALTY_MODELS[name]
+
+
 @dataclass
 class CrsMatrix:
     """Block CRS (src/alg/CrsMatrix.hpp:20-111): rowMap, columnIndices, entries."""
@@ -79,6 +90,8 @@ class ElasticModel:
     graph: tuple | None = None  # optional (rowmap, colidx) host arrays
     device: str = "cuda"
     box: tuple | None = None  # (nx, ny, nz) hexes of a structured box mesh: enables multigrid
+    penalty_model: str = "SIMP"  # "Penalty Function" Type: SIMP, RAMP, Heaviside, NoPenalty
+    regularization_length: float = 1.0  # Heaviside only
 
     def __post_init__(self):
         h = C.c_void_p()
@@ -87,7 +100,8 @@ class ElasticModel:
         _capi.call("pa_create", C.byref(h), self.mesh.num_nodes, self.mesh.num_elems, _hptr(coords), _hptr(conn))
         self._h = h
         _capi.call("pa_set_material", h, float(self.youngs_modulus), float(self.poissons_ratio))
-        _capi.call("pa_set_penalty", h, float(self.penalty_exponent), float(self.penalty_minimum))
+        _capi.call("pa_set_penalty_model", h, penalty_model_id(self.penalty_model), float(self.penalty_exponent),
+                   float(self.penalty_minimum), float(self.regularization_length))
         for bc in self.essential_bcs:
             nodes = np.ascontiguousarray(bc.nodes, np.int32)
             _capi.call("pa_add_essential_bc", h, len(nodes), _hptr(nodes), int(bc.dof), float(bc.value))
@@ -216,6 +230,36 @@ class PhysicsScalarFunction:
         self.m._stream()
         _capi.call("pa_sf_gradient_x", self.m.handle, _dptr(u), _dptr(z), _dptr(g))
         return g
+
+
+class Volume:
+    """Plato::Geometric::Volume (src/geometric/Volume.hpp:49-96) with its own "Penalty
+    Function" (default NoPenalty): V = sum_e vol_e w(mean z_e); gradient_z, gradient_x
+    as GeometryScalarFunction (src/geometric/GeometryScalarFunction.cpp)."""
+
+    def __init__(self, model: ElasticModel, penalty_model: str = "NoPenalty", exponent: float = 1.0,
+                 minimum_value: float = 0.0, regularization_length: float = 1.0):
+        self.m = model
+        self.args = (penalty_model_id(penalty_model), float(exponent), float(minimum_value),
+                     float(regularization_length))
+
+    def _eval(self, z: torch.Tensor, want_z: bool, want_x: bool):
+        v = C.c_double()
+        gz = torch.empty(self.m.num_nodes, dtype=torch.float64, device=z.device) if want_z else None
+        gx = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device) if want_x else None
+        self.m._stream()
+        _capi.call("pa_volume", self.m.handle, _dptr(z), *self.args, C.byref(v),
+                   _dptr(gz) if gz is not None else None, _dptr(gx) if gx is not None else None)
+        return v.value, gz, gx
+
+    def value(self, z: torch.Tensor) -> float:
+        return self._eval(z, False, False)[0]
+
+    def gradient_z(self, z: torch.Tensor) -> torch.Tensor:
+        return self._eval(z, True, False)[1]
+
+    def gradient_x(self, z: torch.Tensor) -> torch.Tensor:
+        return self._eval(z, False, True)[2]
 
 
 class Problem:

@@ -341,3 +341,78 @@ def test_dRdz_T_apply_matches_gradient_z(small):
         for s in range(rm[r], rm[r + 1]):
             want[r] += G[3 * s:3 * s + 3] @ lam[3 * ci[s]:3 * ci[s] + 3]
     np.testing.assert_allclose(model.dRdz_T_apply(u, lam, z), want, atol=1e-12 * np.abs(want).max())
+
+
+# ----------------------------------------------------------------- penalty models, volume
+def test_volume_closed_forms(small):
+    """V = box volume for NoPenalty; linear SIMP (p = 1, rmin = 0) at uniform z = c gives
+    c times it with dV/dz summing to the box volume; translation invariance of dV/dX."""
+    m, z, u, model, rm, ci = small
+    box = 2.0 * 1.0 * 1.0
+    v, gz, gx = model.volume(z, model=3)
+    assert abs(v - box) <= 1e-13 * box
+    np.testing.assert_array_equal(gz, 0.0)
+    np.testing.assert_allclose(gx.reshape(-1, 3).sum(axis=0), 0.0, atol=1e-13)
+    v, gz, gx = model.volume(np.full(m.num_nodes, 0.4), model=0, p=1.0, rmin=0.0)
+    assert abs(v - 0.4 * box) <= 1e-13 * box
+    assert abs(gz.sum() - box) <= 1e-13 * box
+
+
+@pytest.mark.parametrize("pm", [(0, 3.0, 1e-3, 1.0), (1, 3.0, 1e-3, 1.0), (2, 2.0, 1e-3, 1.2), (3, 1.0, 0.0, 1.0)])
+def test_volume_fd(small, pm):
+    m, z, u, model, rm, ci = small
+    kind, p, rmin, L = pm
+    v, gz, gx = model.volume(z, kind, p, rmin, L)
+    for n in [2, m.num_nodes // 3]:
+        h = 1e-6
+        zp, zm = z.copy(), z.copy()
+        zp[n] += h
+        zm[n] -= h
+        fd = (model.volume(zp, kind, p, rmin, L)[0] - model.volume(zm, kind, p, rmin, L)[0]) / (2 * h)
+        assert abs(fd - gz[n]) <= 1e-7 * max(np.abs(gz).max(), 1e-30) + 1e-12, (n, fd, gz[n])
+        for k in range(3):
+            Xp, Xm = m.coords.copy(), m.coords.copy()
+            Xp[n, k] += h
+            Xm[n, k] -= h
+            vp = orc.Model(Xp, m.conn).volume(z, kind, p, rmin, L)[0]
+            vm = orc.Model(Xm, m.conn).volume(z, kind, p, rmin, L)[0]
+            assert abs((vp - vm) / (2 * h) - gx[3 * n + k]) <= 1e-7 * np.abs(gx).max(), (n, k)
+
+
+@pytest.mark.parametrize("pm", [(1, 3.0, 1e-3, 1.0), (2, 2.0, 1e-3, 1.2), (3, 1.0, 0.0, 1.0)])
+def test_penalty_models_fd_and_limits(small, pm):
+    """RAMP, Heaviside and NoPenalty in the residual, Jacobian and energy: dC/dz and dR/dz
+    against central differences; the Jacobian against differences of the residual; and
+    each model's closed form: RAMP(1) = SIMP(1) = 1, Heaviside saturates at 1 for z >= L,
+    NoPenalty equals SIMP with exponent 0."""
+    m, z, u, _, rm, ci = small
+    kind, p, rmin, L = pm
+    mod = orc.Model(m.coords, m.conn, E=1.0, nu=0.3, p=p, rmin=rmin, penalty_model=kind, length=L)
+    v, gu, gz = mod.energy(u, z)
+    G = mod.gradient_z(u, z, rm, ci)
+    K = mod.jacobian(u, z, rm, ci)
+    R0 = mod.residual(u, z)
+    for n in [1, m.num_nodes // 2]:
+        h = 1e-6
+        zp, zm = z.copy(), z.copy()
+        zp[n] += h
+        zm[n] -= h
+        fd = (mod.energy(u, zp, False, False)[0] - mod.energy(u, zm, False, False)[0]) / (2 * h)
+        # floors: rounding of the differenced values (|value| * 1e-16 / h)
+        assert abs(fd - gz[n]) <= 1e-6 * np.abs(gz).max() + 1e-9 * abs(v), (n, fd, gz[n])
+        fdr = (mod.residual(u, zp) - mod.residual(u, zm)) / (2 * h)
+        col = np.zeros(3 * m.num_nodes)
+        for s in range(rm[n], rm[n + 1]):
+            col[3 * ci[s]:3 * ci[s] + 3] = G[3 * s:3 * s + 3]
+        np.testing.assert_allclose(col, fdr, atol=1e-6 * np.abs(fdr).max() + 1e-9 * np.abs(R0).max())
+    # the residual is linear in u: R(u) = K u - f (no traction here)
+    Kd = orc.to_dense(rm, ci, K)
+    np.testing.assert_allclose(Kd @ u, R0, atol=1e-10 * np.abs(R0).max())
+    ones = np.ones(m.num_nodes)
+    simp1 = orc.Model(m.coords, m.conn, E=1.0, nu=0.3, p=1.0, rmin=0.0)
+    if kind in (1, 3):  # RAMP at z = 1 and NoPenalty anywhere: full stiffness
+        zz = ones if kind == 1 else z
+        np.testing.assert_allclose(mod.residual(u, zz), simp1.residual(u, ones), rtol=0, atol=1e-13 * np.abs(R0).max())
+    if kind == 2:  # saturates above the regularization length
+        np.testing.assert_allclose(mod.residual(u, ones * L * 1.01), simp1.residual(u, ones), rtol=0,
+                                   atol=1e-13 * np.abs(R0).max())
