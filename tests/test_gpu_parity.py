@@ -497,3 +497,22 @@ def test_gpu_penalty_models_solves(name, p, L):
         err = np.abs(u - u64).max() / np.abs(u64).max()
         print(f"{name} {prec}/{pre}: {pb.stats.iterations} it, err {err:.1e}", flush=True)
         assert pb.stats.converged and err <= 1e-7
+
+
+def test_gpu_model_from_exodus(tmp_path):
+    """A problem built from an Exodus file's named sets (node set -> essential BCs, side
+    set -> traction) gives the residual and solution of the geometric workload."""
+    from platoanalyze_amd.exodus import model_from_exodus, read_exodus, sides_of_faces, write_exodus
+
+    w = wl.make("cantilever_20x10x10", tiny=(8, 4, 4))
+    el, sd = sides_of_faces(w.mesh, w.face_verts, w.face_elems)
+    path = tmp_path / "cantilever.exo"
+    write_exodus(path, w.mesh, {"x-": w.fixed_nodes}, {"x+": (el, sd)})
+    ex = read_exodus(path)
+    m = model_from_exodus(ex, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min,
+                          essential={"x-": [(0, 0.0), (1, 0.0), (2, 0.0)]}, tractions={"x+": w.traction})
+    ref = wl.model(w)
+    z = dev(w.z)
+    u = dev(np.random.default_rng(2).normal(size=3 * w.mesh.num_nodes))
+    close(host(VectorFunction(m).value(u, z)), host(VectorFunction(ref).value(u, z)), RTOL, "residual")
+    close(host(Problem(m, tolerance=1e-12).solution(z)), host(Problem(ref, tolerance=1e-12).solution(z)), 1e-10, "u")
