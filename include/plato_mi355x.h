@@ -231,6 +231,13 @@ int pa_stress_pnorm(pa_handle h, const double* u, const double* z, int measure, 
  * forming dR/dz. */
 int pa_dRdz_T_apply(pa_handle h, const double* u, const double* lambda, const double* z, double* out);
 
+/* Plato::MatrixTimesVectorPlusVector (src/PlatoMathHelpers.hpp:58-138) on the handle's
+ * block graph: y += alpha * A x.  block_rows 3: 3x3 blocks, 9 values per entry (the
+ * gradient_u and gradient_x layouts, y has 3N entries); block_rows 1: 1x3 blocks, 3
+ * values per entry (the transposed gradient_z layout, y has N entries).  x has 3N. */
+int pa_matvec_plus_vector(pa_handle h, int block_rows, double alpha, const double* values, const double* x,
+                          double* y);
+
 /* Problem::criterionGradient, non-self-adjoint branch (src/elliptic/Problem.hpp:413-468):
  * K lambda = dF/du (homogeneous constraints; FP64 CRS PCG or mixed precision per opts),
  * dF/dz = dF/dz|_u - (dR/dz)^T lambda.  `lambda` (3N) receives the adjoint state. */

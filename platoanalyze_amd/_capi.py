@@ -73,6 +73,7 @@ _SIGS = {
     "pa_compliance_gradient": (_I, [_P, _P, _P, _P, _PD]),
     "pa_stress_pnorm": (_I, [_P, _P, _P, _I, _D, _D, _PD, _P, _P]),
     "pa_dRdz_T_apply": (_I, [_P, _P, _P, _P, _P]),
+    "pa_matvec_plus_vector": (_I, [_P, C.c_int, C.c_double, _P, _P, _P]),
     "pa_stress_pnorm_gradient": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _PD, C.POINTER(SolverOpts),
                                       C.POINTER(SolverStats)]),
     "pa_smooth": (_I, [_P, _P, _P, _I, _D]),

@@ -1547,6 +1547,20 @@ int pa_dRdz_T_apply(pa_handle h, const double* u, const double* lambda, const do
   return PA_OK;
 }
 
+int pa_matvec_plus_vector(pa_handle h, int block_rows, double alpha, const double* values, const double* x,
+                          double* y) {
+  if (int rc = ready(h)) return rc;
+  if (!values || !x || !y) return fail(PA_ERR_ARG, "pa_matvec_plus_vector: null pointer");
+  if (block_rows != 1 && block_rows != 3) return fail(PA_ERR_ARG, "pa_matvec_plus_vector: block_rows must be 1 or 3");
+  const int Gs = pa::red_grid((size_t)h->nn, 16);
+  if (block_rows == 3)
+    pa::k_matvec_add<3><<<Gs, 256, 0, h->stream>>>(values, h->rowmap.p, h->colidx.p, x, y, h->nn, alpha);
+  else
+    pa::k_matvec_add<1><<<Gs, 256, 0, h->stream>>>(values, h->rowmap.p, h->colidx.p, x, y, h->nn, alpha);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
 // Total derivative of the p-norm through K(z) u = f (Problem::criterionGradient, non
 // self-adjoint branch, src/elliptic/Problem.hpp:413-468): K lambda = dF/du with
 // homogeneous constraints, dF/dz = dF/dz|_u - (dR/dz)^T lambda.  lambda (3N, device)

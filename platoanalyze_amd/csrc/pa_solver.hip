@@ -30,6 +30,40 @@ __global__ __launch_bounds__(kT) void k_spmv(const double* __restrict__ vals, co
   }
 }
 
+// y += alpha A x on the block graph for 3x3 blocks (BR = 3: Jacobian and transposed
+// dR/dX layouts, 9 values per entry) or 1x3 blocks (BR = 1: transposed dR/dz, 3 values
+// per entry); 16 lanes per row as in the SpMV.
+template <int BR>
+__global__ __launch_bounds__(kT) void k_matvec_add(const double* __restrict__ vals, const int* __restrict__ rowmap,
+                                                   const int* __restrict__ col, const double* __restrict__ x,
+                                                   double* __restrict__ y, int nrows, double alpha) {
+  const int lane = threadIdx.x % kLanesPerRow;
+  const XcdSpan sp = xcd_span(nrows);
+  for (int row = sp.lo + sp.j * kRowsPerBlock + threadIdx.x / kLanesPerRow; row < sp.hi;
+       row += sp.bx * kRowsPerBlock) {
+    double a[BR];
+#pragma unroll
+    for (int i = 0; i < BR; ++i) a[i] = 0.0;
+    for (int s = rowmap[row] + lane; s < rowmap[row + 1]; s += kLanesPerRow) {
+      const int c = col[s];
+      const double* v = vals + 3 * BR * (size_t)s;
+      const double x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
+#pragma unroll
+      for (int i = 0; i < BR; ++i) a[i] += v[3 * i] * x0 + v[3 * i + 1] * x1 + v[3 * i + 2] * x2;
+    }
+#pragma unroll
+    for (int i = 0; i < BR; ++i) {
+#pragma unroll
+      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) a[i] += __shfl_xor(a[i], off, kLanesPerRow);
+    }
+#pragma unroll
+    for (int i = 0; i < BR; ++i)
+      if (lane == i) y[BR * (size_t)row + i] += alpha * a[i];
+  }
+}
+template __global__ void k_matvec_add<1>(const double*, const int*, const int*, const double*, double*, int, double);
+template __global__ void k_matvec_add<3>(const double*, const int*, const int*, const double*, double*, int, double);
+
 // q = A p; per-group partial of p.q into P_pq.
 __global__ __launch_bounds__(kT) void k_pcg_spmv(const double* __restrict__ vals, const int* __restrict__ rowmap,
                                                  const int* __restrict__ col, const double* __restrict__ p,

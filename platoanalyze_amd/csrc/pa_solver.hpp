@@ -13,6 +13,9 @@ enum PartialSlot { kPPQ = 0, kPRZ0 = 1, kPRZ1 = 2, kPRR = 3, kPBB = 4, kPDot = 5
 
 __global__ void k_spmv(const double* vals, const int* rowmap, const int* col, const double* x,
                        const double* b, double* y, int nrows, int mode);
+template <int BR>
+__global__ void k_matvec_add(const double* vals, const int* rowmap, const int* col, const double* x, double* y,
+                             int nrows, double alpha);
 __global__ void k_pcg_spmv(const double* vals, const int* rowmap, const int* col, const double* p, double* q,
                            int nrows, double* P_pq);
 __global__ void k_pcg_pupdate(double* p, const double* z, int nnodes, const double* P_rz_new,
