@@ -8,7 +8,7 @@ dC/dz) for FP64 and mixed-precision PCG with block-Jacobi and multigrid
 preconditioning, and the CPU baseline (the oracle restatement of the reference
 algorithm on a bounded sample, all host threads and one thread).
 
-    python bench.py [--steps K] [--warmup W] [--config NAME] [--no-cpu] [--no-solve]
+    python bench.py [--steps K] [--warmup W] [--config NAME] [--no-cpu] [--no-solve] [--no-general]
 """
 from __future__ import annotations
 
@@ -73,6 +73,7 @@ def main() -> int:
     ap.add_argument("--config", default="cantilever_128x64x64")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-general", action="store_true", help="skip timing the general chunk kernel")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (for scaling HBM counters)")
@@ -146,19 +147,26 @@ def main() -> int:
         if rc:
             _capi.check(rc, "pa_vf_gradient_u")
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        starts[i].record()
-        step()
-        ends[i].record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / args.steps
+    box = C.c_int(0)
+    _capi.call("pa_box_assembly", h, C.byref(box))
+    kernel_name = ("k_box_w + k_box_jac<true> (lattice Jacobian+residual)" if box.value
+                   else "k_jac_chunk<4,true> (Jacobian+residual)")
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for i in range(steps):
+            starts[i].record()
+            fn()
+            ends[i].record()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sum(s.elapsed_time(e) for s, e in zip(starts, ends)) / steps
+
+    wall, kern_ms = timed(step, args.steps, args.warmup)
     ms_per_step = wall / args.steps * 1e3
     value = E / (wall / args.steps) / 1e6
     B = asm_bytes(E, N, nnzb)
@@ -170,6 +178,28 @@ def main() -> int:
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
 
     extra = {}
+    if box.value and not args.no_general:
+        # the general chunk kernel on the same problem (a second handle without the
+        # lattice kernel), for comparison
+        os.environ["PA_BOX_ASSEMBLY"] = "0"
+        try:
+            mg_ = wl.model(w, device=dev)
+        finally:
+            del os.environ["PA_BOX_ASSEMBLY"]
+        mg_._stream()
+        hg = mg_.handle
+
+        def step_general():
+            rc = L.pa_vf_gradient_u(hg, pu, pz, pv, pr)
+            if rc:
+                _capi.check(rc, "pa_vf_gradient_u")
+
+        _, kg_ms = timed(step_general, args.steps, args.warmup)
+        extra["general_kernel"] = {"kernel": "k_jac_chunk<4,true> (Jacobian+residual)", "kernel_ms": kg_ms,
+                                   "achieved_GBs": B / (kg_ms * 1e-3) / 1e9,
+                                   "frac": B / (kg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del mg_
+        m._stream()
     if not args.no_solve:
         # solve K(z)u = f to 1e-8 relative residual, then C and dC/dz, with each solver:
         # FP64 block-CRS PCG and FP64 PCG with the FP32 matrix-free operator and
@@ -225,7 +255,7 @@ def main() -> int:
                    "step": "gradient_u + fused residual (R = K(z)u - f)", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_jac_chunk<4,true> (Jacobian+residual)", "kernel_ms": kern_ms,
+                     "kernel": kernel_name, "kernel_ms": kern_ms,
                      "algorithmic_bytes": B},
         "cpu_baseline": cpu,
         "setup_s": {"mesh": t_mesh, "graph_chunks_upload": t_setup},

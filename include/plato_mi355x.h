@@ -201,6 +201,14 @@ int pa_solution(pa_handle h, const double* z, double* u, double* values,
  * src/alg/EpetraLinearSolver.cpp); PA_ERR_MESH when the nodes are not such a lattice. */
 int pa_set_box(pa_handle h, int nx, int ny, int nz);
 
+/* No counterpart in the reference (an introspection call for callers and tests): *active
+ * = 1 when the handle assembles its Jacobian with the lattice kernel, which applies after
+ * pa_set_box when every hex of the box carries the same six lattice tets, the nodes sit
+ * on the uniform lattice (within 1e-13 spacings) and every graph row has at most 16
+ * blocks; the values equal the general kernel's to rounding.  PA_BOX_ASSEMBLY=0 in the
+ * environment at pa_set_box / pa_finalize keeps the general kernel. */
+int pa_box_assembly(pa_handle h, int* active);
+
 /* Damped block-Jacobi smoothing of the homogeneous problem, u <- u - omega D^-1 K(z) u
  * at the free dofs, `steps` times (D: 3x3 diagonal blocks of K).  A low-pass filter
  * adapted to K: modes with a large Rayleigh quotient are damped, smooth ones kept.  No

@@ -78,6 +78,7 @@ _SIGS = {
                                       C.POINTER(SolverStats)]),
     "pa_smooth": (_I, [_P, _P, _P, _I, _D]),
     "pa_set_box": (_I, [_P, _I, _I, _I]),
+    "pa_box_assembly": (_I, [_P, _PIN]),
     "pa_set_penalty_model": (_I, [_P, _I, _D, _D, _D]),
     "pa_volume": (_I, [_P, _P, _I, _D, _D, _D, _PD, _P, _P]),
     "pa_unet_last_error": (C.c_char_p, []),

@@ -1,0 +1,66 @@
+// Jacobian assembly on a uniform box lattice (every hex split into the same six tets).
+//
+// On such a mesh each element is one of six lattice tets, so its stiffness block for a
+// vertex pair (a, b) is w_e * B[t][a][b], with B the unit-weight block of lattice tet t
+// (vol (lam g_a g_b^T + mu g_b g_a^T + mu (g_a.g_b) I)) and w_e the penalised density
+// weight of the element.  A graph block K_rc is then a weighted sum of at most a few
+// constant blocks: the elements around the edge (r, c) are fixed lattice offsets from r.
+// The kernel needs no connectivity, coordinates or element records; it reads the graph,
+// one packed lattice position per node and the element weights, and writes each CRS
+// value once.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pa_kernels.hpp"
+
+namespace pa {
+
+constexpr int kBoxMaxRow = 16;     // graph row length limit (one 16-lane group per row)
+constexpr int kBoxRows = 16;       // rows per 256-thread work-group
+constexpr int kBoxMaxTerms = 6;    // element terms per off-diagonal block
+constexpr int kBoxDiagLane = 15;   // stencil lane of the diagonal block
+constexpr int kBoxMaxAxis = 1024;  // nodes per axis (10-bit packed lattice positions)
+
+// Per stencil neighbour l (< 15; lane 15 is the diagonal): its lattice offset, the
+// number of element terms and, per term e, the element-weight offset from the row's
+// padded hex index (t * nhex - hex offset) and the unit-weight block P of (t, a, b).
+struct BoxStencil {
+  int count[16];
+  int off[16][kBoxMaxTerms];
+  int8_t d[16][3];
+};
+
+struct BoxArgs {
+  const int* rowmap;
+  const int* col;
+  const int* gpack;        // per node: i | j << 10 | k << 20 (lattice position)
+  const unsigned long long* perm;  // per row: CRS slot of stencil lane l in bits 4l..4l+3
+  const int* nog;          // node at lattice point i + (nx+1) (j + (ny+1) k)
+  const uint8_t* corners;  // [6][4] corner of each local vertex of the lattice tets (bit 0 x, 1 y, 2 z)
+  const double* z;
+  double* W;               // [6][(nx+2)(ny+2)(nz+2)] element weights, zero on the ghost layer
+  const double* B;         // [16][6][9] material blocks per stencil lane and term at unit weight
+  const BoxStencil* S;
+  const double* u;
+  const int* lidx;         // per node: its entry in the load list, or -1
+  const double* load_f;    // [3 * loads] nodal loads f
+  double* vals;
+  double* R;
+  int nrows;
+  int nx, ny, nz;          // hexes per axis
+  Material m;
+};
+
+// W for every padded hex (one thread per hex; ghost hexes get zero weights).
+__global__ void k_box_w(BoxArgs A);
+
+// B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T.
+__global__ void k_box_material(const double* P, double* B, double lam, double mu);
+
+// K values of every graph row (and R = K u - f when WITH_R); one work-group per 16 rows.
+template <bool WITH_R>
+__global__ void k_box_jac(BoxArgs A);
+
+}  // namespace pa

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +13,7 @@
 
 #include "../../include/plato_mi355x.h"
 #include "pa_host.hpp"
+#include "pa_box.hpp"
 #include "pa_kernels.hpp"
 #include "pa_mf.hpp"
 #include "pa_mg.hpp"
@@ -147,6 +149,17 @@ struct pa_problem {
   int mg_pre0 = 1, mg_post0 = 1, mg_degc = 2;
   double mg_ratio = 60.0;
   DevArray<double> g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
+  // lattice assembly (pa_box.hip): enabled by box_fast_setup when the box's hexes all
+  // carry the same six lattice tets, the nodes sit on the uniform lattice and the graph
+  // rows fit 16 blocks inside the 27-point stencil
+  bool box_fast = false;
+  DevArray<int32_t> box_gpack, box_nog;
+  DevArray<uint8_t> box_corners;
+  DevArray<int32_t> box_lidx;
+  DevArray<unsigned long long> box_perm;
+  DevArray<double> box_P, box_B, box_W;
+  DevArray<pa::BoxStencil> box_S;
+  double box_lam = 0.0, box_mu = 0.0;  // material of box_B (rebuilt when it changes)
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -197,6 +210,9 @@ struct pa_problem {
 MgLevel::~MgLevel() {
   if (owned) delete h;
 }
+
+static int box_fast_setup(pa_handle h);
+static int assemble_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R);
 
 static int grid_for(size_t n, int per_block = 256, int cap = 4096) {
   size_t g = (n + per_block - 1) / per_block;
@@ -447,6 +463,8 @@ int pa_finalize(pa_handle h) {
   PA_HIP(h->chunk_partials.alloc(std::max(1, h->chunks.count)));
   PA_HIP(hipHostMalloc(&h->host_scal, pa::kNumScal * sizeof(double), hipHostMallocDefault));
   h->finalized = true;
+  if (h->has_box)
+    if (int rc = box_fast_setup(h)) return rc;
   return PA_OK;
 }
 
@@ -502,8 +520,7 @@ int pa_vf_value(pa_handle h, const double* u, const double* z, double* R) {
 int pa_vf_gradient_u(pa_handle h, const double* u, const double* z, double* vals, double* R) {
   if (int rc = ready(h)) return rc;
   if (!z || !vals || (R && !u)) return fail(PA_ERR_ARG, "pa_vf_gradient_u: null pointer");
-  if (R) return run_chunk(h, pa::kModeJacobianResidual, u, z, vals, R, true);
-  return run_chunk(h, pa::kModeJacobian, u, z, vals, nullptr);
+  return assemble_jacobian(h, u, z, vals, R);
 }
 
 int pa_vf_gradient_z(pa_handle h, const double* u, const double* z, double* vals_t) {
@@ -955,7 +972,242 @@ int pa_set_box(pa_handle h, int nx, int ny, int nz) {
   h->box_grid.swap(grid);
   h->has_box = true;
   h->mg.clear();
+  if (h->finalized) return box_fast_setup(h);
   return PA_OK;
+}
+
+// Host replica of the element gradient formula (tet_gradients in pa_kernels.hip).
+static void host_tet_gradients(const double x[4][3], double g[4][3], double& vol) {
+  const double j11 = x[0][0] - x[3][0], j12 = x[1][0] - x[3][0], j13 = x[2][0] - x[3][0];
+  const double j21 = x[0][1] - x[3][1], j22 = x[1][1] - x[3][1], j23 = x[2][1] - x[3][1];
+  const double j31 = x[0][2] - x[3][2], j32 = x[1][2] - x[3][2], j33 = x[2][2] - x[3][2];
+  const double det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 -
+                     j12 * j21 * j33 - j13 * j22 * j31;
+  const double inv = 1.0 / det;
+  g[0][0] = (j22 * j33 - j23 * j32) * inv;
+  g[0][1] = (j13 * j32 - j12 * j33) * inv;
+  g[0][2] = (j12 * j23 - j13 * j22) * inv;
+  g[1][0] = (j23 * j31 - j21 * j33) * inv;
+  g[1][1] = (j11 * j33 - j13 * j31) * inv;
+  g[1][2] = (j13 * j21 - j11 * j23) * inv;
+  g[2][0] = (j21 * j32 - j22 * j31) * inv;
+  g[2][1] = (j12 * j31 - j11 * j32) * inv;
+  g[2][2] = (j11 * j22 - j12 * j21) * inv;
+  for (int d = 0; d < 3; ++d) g[3][d] = -(g[0][d] + g[1][d] + g[2][d]);
+  vol = std::fabs(det) * (1.0 / 6.0);
+}
+
+// Decide whether the lattice assembly applies and build its tables.  Conditions: at most
+// 1024 nodes per axis; every node within 1e-13 spacings of its lattice point; every
+// element inside one hex, the elements of every hex being the same six corner sets; every
+// graph row at most 16 blocks, all within one lattice step of the row's node.
+// PA_BOX_ASSEMBLY=0 keeps the general chunk kernel.
+static int box_fast_setup(pa_handle h) {
+  h->box_fast = false;
+  if (const char* e = std::getenv("PA_BOX_ASSEMBLY"))
+    if (std::atoi(e) == 0) return PA_OK;
+  const pa::BoxDims d = h->box;
+  if (d.nx > pa::kBoxMaxAxis || d.ny > pa::kBoxMaxAxis || d.nz > pa::kBoxMaxAxis) return PA_OK;
+  if (h->graph.max_row_len > pa::kBoxMaxRow) return PA_OK;
+  const int hx = d.nx - 1, hy = d.ny - 1, hz = d.nz - 1;
+  if (hx < 1 || hy < 1 || hz < 1 || (size_t)6 * hx * hy * hz != (size_t)h->ne) return PA_OK;
+  const double step[3] = {h->box_len[0] / hx, h->box_len[1] / hy, h->box_len[2] / hz};
+  const std::vector<int32_t>& gr = h->box_grid;
+  for (int n = 0; n < h->nn; ++n)
+    for (int a = 0; a < 3; ++a) {
+      const double t = (h->coords[3 * (size_t)n + a] - h->box_lo[a]) / step[a];
+      if (std::fabs(t - gr[3 * (size_t)n + a]) > 1e-13) return PA_OK;
+    }
+  // element types: sorted corner ids (bit 0 x, bit 1 y, bit 2 z) relative to the hex
+  std::vector<std::array<uint8_t, 4>> types;
+  std::vector<uint8_t> seen((size_t)hx * hy * hz, 0);
+  for (int e = 0; e < h->ne; ++e) {
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30};
+    for (int a = 0; a < 4; ++a)
+      for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], gr[3 * (size_t)h->conn[4 * (size_t)e + a] + k]);
+    std::array<uint8_t, 4> cs;
+    for (int a = 0; a < 4; ++a) {
+      int c = 0;
+      for (int k = 0; k < 3; ++k) {
+        const int o = gr[3 * (size_t)h->conn[4 * (size_t)e + a] + k] - lo[k];
+        if (o < 0 || o > 1) return PA_OK;
+        c |= o << k;
+      }
+      cs[a] = (uint8_t)c;
+    }
+    std::sort(cs.begin(), cs.end());
+    if (cs[0] == cs[1] || cs[1] == cs[2] || cs[2] == cs[3]) return PA_OK;
+    if (lo[0] >= hx || lo[1] >= hy || lo[2] >= hz) return PA_OK;
+    int t = 0;
+    while (t < (int)types.size() && types[t] != cs) ++t;
+    if (t == (int)types.size()) {
+      if (types.size() == 6) return PA_OK;
+      types.push_back(cs);
+    }
+    uint8_t& m = seen[((size_t)lo[2] * hy + lo[1]) * hx + lo[0]];
+    if (m & (1u << t)) return PA_OK;
+    m |= (uint8_t)(1u << t);
+  }
+  if (types.size() != 6) return PA_OK;
+  for (uint8_t m : seen)
+    if (m != 0x3f) return PA_OK;
+  const pa::Graph& g = h->graph;
+  // unit-weight blocks of each lattice tet: Pt[t][a][b][i][j] = vol g_a[i] g_b[j]
+  std::vector<double> Pt(6 * 4 * 4 * 9);
+  std::vector<uint8_t> corners(6 * 4);
+  for (int t = 0; t < 6; ++t) {
+    double x[4][3], gg[4][3], vol;
+    for (int a = 0; a < 4; ++a) {
+      corners[4 * t + a] = types[t][a];
+      for (int k = 0; k < 3; ++k) x[a][k] = ((types[t][a] >> k) & 1) * step[k];
+    }
+    host_tet_gradients(x, gg, vol);
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) Pt[9 * ((t * 4 + a) * 4 + b) + 3 * i + j] = vol * gg[a][i] * gg[b][j];
+  }
+  // Stencil: the element terms of K_rc per lattice offset c - r.  The row node is corner
+  // hc of the hex whose lower corner is its lattice point minus the bits of hc; in tet t
+  // of that hex it is local vertex a, the column node local vertex b.
+  const size_t nhex = (size_t)(hx + 2) * (hy + 2) * (hz + 2);
+  if (6 * nhex >= (size_t)1 << 31) return PA_OK;
+  const int px = hx + 2, py = hy + 2;
+  std::vector<std::vector<std::array<int, 4>>> terms(27);  // (hc, t, a, b) per offset
+  for (int hc = 0; hc < 8; ++hc)
+    for (int t = 0; t < 6; ++t)
+      for (int a = 0; a < 4; ++a) {
+        if (types[t][a] != hc) continue;
+        for (int b = 0; b < 4; ++b) {
+          if (b == a) continue;
+          const int cb = types[t][b];
+          int di = 0, mul = 1;
+          for (int k = 0; k < 3; ++k, mul *= 3) di += (((cb >> k) & 1) - ((hc >> k) & 1) + 1) * mul;
+          terms[di].push_back({hc, t, a, b});
+        }
+      }
+  pa::BoxStencil st{};
+  std::vector<double> P(16 * pa::kBoxMaxTerms * 9, 0.0);
+  int nst = 0;
+  for (int di = 0; di < 27; ++di) {
+    if (terms[di].empty()) continue;
+    if (di == 13 || nst >= pa::kBoxDiagLane || terms[di].size() > (size_t)pa::kBoxMaxTerms) return PA_OK;
+    const int l = nst++;
+    st.count[l] = (int)terms[di].size();
+    st.d[l][0] = (int8_t)(di % 3 - 1);
+    st.d[l][1] = (int8_t)((di / 3) % 3 - 1);
+    st.d[l][2] = (int8_t)(di / 9 - 1);
+    for (size_t e = 0; e < terms[di].size(); ++e) {
+      const auto& tm = terms[di][e];
+      const int hc = tm[0], t = tm[1], a = tm[2], b = tm[3];
+      st.off[l][e] = (int)(t * nhex) - ((hc & 1) + px * ((hc >> 1) & 1) + px * py * (hc >> 2));
+      for (int q = 0; q < 9; ++q) P[9 * (l * pa::kBoxMaxTerms + e) + q] = Pt[9 * ((t * 4 + a) * 4 + b) + q];
+    }
+  }
+  // every graph row holds exactly its node and its present stencil neighbours; the slot
+  // word of a row holds the CRS position of each stencil lane's block (4 bits per lane,
+  // lane 15 the diagonal)
+  std::vector<int32_t> gpack(h->nn), nog((size_t)d.nx * d.ny * d.nz);
+  for (int n = 0; n < h->nn; ++n) {
+    const int i = gr[3 * (size_t)n], j = gr[3 * (size_t)n + 1], k = gr[3 * (size_t)n + 2];
+    gpack[n] = i | j << 10 | k << 20;
+    nog[i + (size_t)d.nx * (j + (size_t)d.ny * k)] = n;
+  }
+  std::vector<unsigned long long> perm(h->nn, 0ull);
+  for (int r = 0; r < g.num_rows; ++r) {
+    const int s0 = g.rowmap[r], len = g.rowmap[r + 1] - s0;
+    if (len > pa::kBoxMaxRow) return PA_OK;
+    int found = 0;
+    auto place = [&](int node, int lane) {
+      for (int s = 0; s < len; ++s)
+        if (g.colidx[s0 + s] == node) {
+          perm[r] |= (unsigned long long)s << (4 * lane);
+          ++found;
+          return true;
+        }
+      return false;
+    };
+    if (!place(r, pa::kBoxDiagLane)) return PA_OK;
+    for (int l = 0; l < nst; ++l) {
+      const int i = gr[3 * (size_t)r] + st.d[l][0], j = gr[3 * (size_t)r + 1] + st.d[l][1],
+                k = gr[3 * (size_t)r + 2] + st.d[l][2];
+      if (i < 0 || j < 0 || k < 0 || i >= d.nx || j >= d.ny || k >= d.nz) continue;
+      if (!place(nog[i + (size_t)d.nx * (j + (size_t)d.ny * k)], l)) return PA_OK;
+    }
+    if (found != len) return PA_OK;  // (graph rows hold no repeated columns)
+  }
+  // load-list entry per node (the list holds the nodes with a nonzero load, ascending)
+  std::vector<int32_t> lidx(h->nn, -1);
+  for (int n = 0, k = 0; n < h->nn; ++n) {
+    const double* fn = &h->f[3 * (size_t)n];
+    if (fn[0] != 0.0 || fn[1] != 0.0 || fn[2] != 0.0) lidx[n] = k++;
+  }
+  PA_HIP(h->box_lidx.upload(lidx));
+  PA_HIP(h->box_B.alloc(P.size()));
+  h->box_lam = h->box_mu = 0.0;
+  PA_HIP(h->box_gpack.upload(gpack));
+  PA_HIP(h->box_perm.upload(perm));
+  PA_HIP(h->box_nog.upload(nog));
+  PA_HIP(h->box_corners.upload(corners));
+  PA_HIP(h->box_P.upload(P));
+  PA_HIP(h->box_S.upload(std::vector<pa::BoxStencil>{st}));
+  PA_HIP(h->box_W.alloc((size_t)6 * (hx + 2) * (hy + 2) * (hz + 2)));
+  h->box_fast = true;
+  return PA_OK;
+}
+
+int pa_box_assembly(pa_handle h, int* active) {
+  if (!h || !active) return fail(PA_ERR_ARG, "pa_box_assembly: null argument");
+  *active = h->box_fast ? 1 : 0;
+  return PA_OK;
+}
+
+// K (and R = K u - f) on the lattice: element weights, then the rows.
+static int box_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R) {
+  pa::BoxArgs a{};
+  a.rowmap = h->rowmap.p;
+  a.col = h->colidx.p;
+  a.gpack = h->box_gpack.p;
+  a.perm = h->box_perm.p;
+  a.nog = h->box_nog.p;
+  a.corners = h->box_corners.p;
+  a.z = z;
+  a.W = h->box_W.p;
+  a.B = h->box_B.p;
+  a.S = h->box_S.p;
+  a.u = u;
+  a.lidx = h->box_lidx.p;
+  a.load_f = h->load_f.p;
+  a.vals = vals;
+  a.R = R;
+  a.nrows = h->nn;
+  a.nx = h->box.nx - 1;
+  a.ny = h->box.ny - 1;
+  a.nz = h->box.nz - 1;
+  a.m = h->material();
+  if (a.m.c01 != h->box_lam || a.m.c33 != h->box_mu) {
+    pa::k_box_material<<<1, 256, 0, h->stream>>>(h->box_P.p, h->box_B.p, a.m.c01, a.m.c33);
+    h->box_lam = a.m.c01;
+    h->box_mu = a.m.c33;
+  }
+  const size_t nh = (size_t)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
+  pa::k_box_w<<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
+  const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
+  const int G = (nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+  if (R)
+    pa::k_box_jac<true><<<G, 256, 0, h->stream>>>(a);
+  else
+    pa::k_box_jac<false><<<G, 256, 0, h->stream>>>(a);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+// The Jacobian (and optionally R = K u - f) through the lattice kernel when it applies,
+// else the chunk kernel.
+static int assemble_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R) {
+  if (h->box_fast) return box_jacobian(h, u, z, vals, R);
+  if (R) return run_chunk(h, pa::kModeJacobianResidual, u, z, vals, R, true);
+  return run_chunk(h, pa::kModeJacobian, u, z, vals, nullptr);
 }
 
 static int mg_level_init(MgLevel& L, const std::vector<int32_t>& grid) {
@@ -1485,7 +1737,7 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   rhs = h->w_vals.p;
   pa::k_fill<<<grid_for(ndof), 256, 0, h->stream>>>(rhs, ndof, 0.0);
   if (int rc = sub_load(h, rhs, 1.0)) return rc;
-  if (int rc = run_chunk(h, pa::kModeJacobian, nullptr, z, vals, nullptr)) return rc;
+  if (int rc = assemble_jacobian(h, nullptr, z, vals, nullptr)) return rc;
   if (int rc = pa_apply_constraints(h, vals, rhs, 1.0)) return rc;
   opts.use_x0 = 0;
   if (mg) return solve_mg(h, z, u, rhs, vals, opts, st, false);
@@ -1587,7 +1839,7 @@ int pa_stress_pnorm_gradient(pa_handle h, const double* u, const double* z, int 
   } else {
     const size_t nv = 9 * (size_t)h->graph.rowmap[h->nn];
     if (h->w_K.n != nv) PA_HIP(h->w_K.alloc(nv));
-    if (int rc = run_chunk(h, pa::kModeJacobian, nullptr, z, h->w_K.p, nullptr)) return rc;
+    if (int rc = assemble_jacobian(h, nullptr, z, h->w_K.p, nullptr)) return rc;
     if (int rc = pa_apply_constraints(h, h->w_K.p, dfdu, 0.0)) return rc;  // homogeneous on lambda
     if (int rc = mg ? solve_mg(h, z, lambda, dfdu, h->w_K.p, opts, st, false) : pa_solve(h, h->w_K.p, lambda, dfdu, &opts, st))
       return rc;
