@@ -29,6 +29,7 @@ constexpr int kBoxMaxAxis = 1024;  // nodes per axis (10-bit packed lattice posi
 struct BoxStencil {
   int count[16];
   int off[16][kBoxMaxTerms];
+  int uoff[16];        // padded-lattice offset of the neighbour: d0 + px (d1 + py d2)
   int8_t d[16][3];
 };
 
@@ -37,13 +38,14 @@ struct BoxArgs {
   const int* col;
   const int* gpack;        // per node: i | j << 10 | k << 20 (lattice position)
   const unsigned long long* perm;  // per row: CRS slot of stencil lane l in bits 4l..4l+3
-  const int* nog;          // node at lattice point i + (nx+1) (j + (ny+1) k)
   const uint8_t* corners;  // [6][4] corner of each local vertex of the lattice tets (bit 0 x, 1 y, 2 z)
   const double* z;
   double* W;               // [6][(nx+2)(ny+2)(nz+2)] element weights, zero on the ghost layer
   const double* B;         // [16][6][9] material blocks per stencil lane and term at unit weight
   const BoxStencil* S;
   const double* u;
+  double* ulat;            // [3][padded lattice] u at node lattice points (padded like the hexes)
+  double* zlat;            // [padded lattice] z at node lattice points
   const int* lidx;         // per node: its entry in the load list, or -1
   const double* load_f;    // [3 * loads] nodal loads f
   double* vals;
@@ -53,7 +55,12 @@ struct BoxArgs {
   Material m;
 };
 
-// W for every padded hex (one thread per hex; ghost hexes get zero weights).
+// z (and u when ulat is set) of every node at its padded lattice position (one thread
+// per node: coalesced reads, scattered writes).
+__global__ void k_box_scatter(BoxArgs A);
+
+// W for every padded hex from the lattice-ordered z (one thread per hex; ghost hexes get
+// zero weights).
 __global__ void k_box_w(BoxArgs A);
 
 // B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T.

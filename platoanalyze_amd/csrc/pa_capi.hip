@@ -153,13 +153,14 @@ struct pa_problem {
   // carry the same six lattice tets, the nodes sit on the uniform lattice and the graph
   // rows fit 16 blocks inside the 27-point stencil
   bool box_fast = false;
-  DevArray<int32_t> box_gpack, box_nog;
+  DevArray<int32_t> box_gpack;
   DevArray<uint8_t> box_corners;
   DevArray<int32_t> box_lidx;
   DevArray<unsigned long long> box_perm;
-  DevArray<double> box_P, box_B, box_W;
+  DevArray<double> box_P, box_B, box_W, box_ulat, box_zlat;
   DevArray<pa::BoxStencil> box_S;
   double box_lam = 0.0, box_mu = 0.0;  // material of box_B (rebuilt when it changes)
+  int box_grid_k = 1024, box_grid_r = 768;  // persistent grids of the lattice kernel (K, K and R)
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -1097,6 +1098,7 @@ static int box_fast_setup(pa_handle h) {
     st.d[l][0] = (int8_t)(di % 3 - 1);
     st.d[l][1] = (int8_t)((di / 3) % 3 - 1);
     st.d[l][2] = (int8_t)(di / 9 - 1);
+    st.uoff[l] = st.d[l][0] + px * (st.d[l][1] + py * st.d[l][2]);
     for (size_t e = 0; e < terms[di].size(); ++e) {
       const auto& tm = terms[di][e];
       const int hc = tm[0], t = tm[1], a = tm[2], b = tm[3];
@@ -1147,11 +1149,26 @@ static int box_fast_setup(pa_handle h) {
   h->box_lam = h->box_mu = 0.0;
   PA_HIP(h->box_gpack.upload(gpack));
   PA_HIP(h->box_perm.upload(perm));
-  PA_HIP(h->box_nog.upload(nog));
   PA_HIP(h->box_corners.upload(corners));
   PA_HIP(h->box_P.upload(P));
   PA_HIP(h->box_S.upload(std::vector<pa::BoxStencil>{st}));
   PA_HIP(h->box_W.alloc((size_t)6 * (hx + 2) * (hy + 2) * (hz + 2)));
+  PA_HIP(h->box_ulat.alloc((size_t)3 * (hx + 2) * (hy + 2) * (hz + 2)));
+  PA_HIP(h->box_zlat.alloc((size_t)(hx + 2) * (hy + 2) * (hz + 2)));
+  {  // persistent grids: the resident work-groups per CU of each variant, times the CUs
+    int dev = 0, cus = 256, bk = 4, br = 3;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && cus > 0) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bk, pa::k_box_jac<false>, 256, 0) != hipSuccess || bk < 1)
+        bk = 4;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&br, pa::k_box_jac<true>, 256, 0) != hipSuccess || br < 1)
+        br = 3;
+    }
+    h->box_grid_k = (bk * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+    h->box_grid_r = (br * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+    if (const char* e = std::getenv("PA_BOX_BLOCKS"))
+      if (std::atoi(e) >= pa::kNumXcd) h->box_grid_k = h->box_grid_r = std::atoi(e) / pa::kNumXcd * pa::kNumXcd;
+  }
   h->box_fast = true;
   return PA_OK;
 }
@@ -1169,13 +1186,14 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
   a.col = h->colidx.p;
   a.gpack = h->box_gpack.p;
   a.perm = h->box_perm.p;
-  a.nog = h->box_nog.p;
   a.corners = h->box_corners.p;
   a.z = z;
   a.W = h->box_W.p;
   a.B = h->box_B.p;
   a.S = h->box_S.p;
   a.u = u;
+  a.ulat = R ? h->box_ulat.p : nullptr;
+  a.zlat = h->box_zlat.p;
   a.lidx = h->box_lidx.p;
   a.load_f = h->load_f.p;
   a.vals = vals;
@@ -1191,9 +1209,10 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
     h->box_mu = a.m.c33;
   }
   const size_t nh = (size_t)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
+  pa::k_box_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, h->stream>>>(a);
   pa::k_box_w<<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
   const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
-  const int G = (nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+  const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, R ? h->box_grid_r : h->box_grid_k);
   if (R)
     pa::k_box_jac<true><<<G, 256, 0, h->stream>>>(a);
   else
