@@ -61,21 +61,30 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
 // staged in LDS and streamed out with coalesced non-temporal stores.  Work-groups are
 // dealt to the XCDs in contiguous row ranges (round-robin dispatch puts group b on XCD
 // b % 8) and looped over within each XCD's range (a persistent grid: the block table
-// and the lane constants are loaded once per work-group).  The loop is software-
-// pipelined: while a tile is computed and streamed out, the next tile's gathers and the
-// tile after's row data are in flight.
+// and the lane constants are loaded once per work-group). 
 // Work-group barrier for LDS hand-offs only: waits for this wave's LDS operations, not
-// for its outstanding global loads (the next tiles' prefetches stay in flight).
+// for its outstanding global stores.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-struct BoxTile {  // a tile's row-level data (first-level loads)
-  int base, last, s_base, s_end, s0, gr;
-  unsigned long long pm;
-};
-struct BoxGath {  // a lane's gathered inputs (second-level loads)
-  double w[kBoxMaxTerms];
-  double ur[3], uc[3];
-};
+// Sums of 12 values over each aligned 16-lane group, scattered over the lanes (a
+// reduce-scatter in four DPP exchanges): lane i receives the group sum of value index
+// rs_index(i), or nothing when rs_index(i) < 0.  Each exchange pairs every lane with a
+// partner holding the complementary half of the current value set.
+__device__ __forceinline__ int rs_index(int lane) {
+  const int o = ((lane & 2) ? 2 : 0) + (lane & 1);  // offset within the last three values
+  return o > 2 ? -1 : (lane < 8 ? 0 : 6) + ((lane & 7) < 4 ? 0 : 3) + o;
+}
+__device__ __forceinline__ double reduce_scatter12(const double v[12], int lane) {
+  const bool l1 = lane < 8, l2 = (lane & 7) < 4, l3 = (lane & 2) == 0, l4 = (lane & 1) == 0;
+  double a[6], b[3];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a[k] = (l1 ? v[k] : v[6 + k]) + dpp_f64<kDppMirror>(l1 ? v[6 + k] : v[k]);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) b[k] = (l2 ? a[k] : a[3 + k]) + dpp_f64<kDppHalfMirror>(l2 ? a[3 + k] : a[k]);
+  const double c0 = (l3 ? b[0] : b[2]) + dpp_f64<kDppXor2>(l3 ? b[2] : b[0]);
+  const double c1 = (l3 ? b[1] : 0.0) + dpp_f64<kDppXor2>(l3 ? 0.0 : b[1]);
+  return (l4 ? c0 : c1) + dpp_f64<kDppXor1>(l4 ? c1 : c0);
+}
 
 template <bool WITH_R>
 __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
@@ -85,129 +94,95 @@ __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
   const int lane = threadIdx.x % 16, grp = threadIdx.x / 16;
   // per-lane stencil constants, kept in registers for every tile
   const int cnt = lane == kBoxDiagLane ? 0 : A.S->count[lane];
-  int off[kBoxMaxTerms];
+  unsigned off[kBoxMaxTerms];
 #pragma unroll
-  for (int e = 0; e < kBoxMaxTerms; ++e) off[e] = A.S->off[lane][e];
+  for (int e = 0; e < kBoxMaxTerms; ++e) off[e] = (unsigned)A.S->off[lane][e];
   const int d0 = A.S->d[lane][0], d1 = A.S->d[lane][1], d2 = A.S->d[lane][2];
-  const int uoff = A.S->uoff[lane];
-  const int px = A.nx + 2, py = A.ny + 2;
-  const size_t nh = (size_t)px * py * (A.nz + 2);
+  const unsigned uoff = (unsigned)A.S->uoff[lane];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const unsigned nh = px * py * (unsigned)(A.nz + 2);
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ ul = A.ulat;
   __syncthreads();  // block table in LDS
   const double* Bl = sB + lane * (kBoxMaxTerms * 9);
   const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
   const int per = (nblk + kNumXcd - 1) / kNumXcd;
   const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
   const int t_end = min((x + 1) * per, nblk);
-  int blk = x * per + (int)blockIdx.x / kNumXcd;
-  if (blk >= t_end) return;  // uniform over the work-group
-
-  auto load1 = [&](int b, BoxTile& T) {
-    T.base = b * kBoxRows;
-    T.last = min(T.base + kBoxRows, A.nrows);
-    T.s_base = A.rowmap[T.base];
-    T.s_end = A.rowmap[T.last];
-    const int row = T.base + grp;
-    T.s0 = 0;
-    T.gr = -1;
-    T.pm = 0;
-    if (row < T.last) {
-      T.s0 = A.rowmap[row];
-      T.gr = A.gpack[row];
-      T.pm = A.perm[row];
+  for (int blk = x * per + (int)blockIdx.x / kNumXcd; blk < t_end; blk += nbx) {
+    const int base = blk * kBoxRows;
+    const int row = base + grp;
+    const int last = min(base + kBoxRows, A.nrows);
+    const int s_base = A.rowmap[base], s_end = A.rowmap[last];
+    const bool live = row < last;
+    int s0 = 0, gr = 0;
+    unsigned long long pm = 0;
+    if (live) {
+      s0 = A.rowmap[row];
+      gr = A.gpack[row];
+      pm = A.perm[row];
     }
-  };
-  // the lane's stencil neighbour is present when inside the lattice (the host checked
-  // that every row holds exactly the present neighbours and itself)
-  auto present = [&](const BoxTile& T) {
-    const int ri = T.gr & 1023, rj = (T.gr >> 10) & 1023, rk = T.gr >> 20;
-    return T.gr >= 0 && cnt > 0 && (unsigned)(ri + d0) <= (unsigned)A.nx && (unsigned)(rj + d1) <= (unsigned)A.ny &&
-           (unsigned)(rk + d2) <= (unsigned)A.nz;
-  };
-  auto lattice = [&](const BoxTile& T) {  // padded index of the row's lattice point
-    const int ri = T.gr & 1023, rj = (T.gr >> 10) & 1023, rk = T.gr >> 20;
-    return (ri + 1) + px * ((rj + 1) + py * (rk + 1));
-  };
-  auto load2 = [&](const BoxTile& T, BoxGath& G) {
-    const bool nb = present(T);
-    const int hr = lattice(T);
-#pragma unroll
-    for (int e = 0; e < kBoxMaxTerms; ++e) G.w[e] = (nb && e < cnt) ? A.W[hr + off[e]] : 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      G.ur[i] = 0.0;
-      G.uc[i] = 0.0;
-    }
-    if constexpr (WITH_R) {
-      if (nb) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          G.ur[i] = A.ulat[i * nh + hr];
-          G.uc[i] = A.ulat[i * nh + hr + uoff];
-        }
-      }
-    }
-  };
-
-  BoxTile T0, T1;
-  BoxGath G0;
-  load1(blk, T0);
-  load2(T0, G0);
-  bool has1 = blk + nbx < t_end;
-  if (has1) load1(blk + nbx, T1);
-  for (; blk < t_end; blk += nbx) {
-    // T0/G0: this tile; T1: the next tile's rows (when has1)
-    BoxTile T2;
-    BoxGath G1;
-    if (has1) load2(T1, G1);
-    const bool has2 = blk + 2 * nbx < t_end;
-    if (has2) load1(blk + 2 * nbx, T2);
-
-    const int row = T0.base + grp;
-    const bool live = row < T0.last;
-    const bool nb = present(T0);
-    const bool diag = live && lane == kBoxDiagLane;
-    const int slot = (int)((T0.pm >> (4 * lane)) & 15);
+    const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
+    const int slot = (int)((pm >> (4 * lane)) & 15);
+    // this lane's stencil neighbour: present when inside the lattice (the host checked
+    // that every row holds exactly the present neighbours and itself)
+    const bool nb = live && cnt > 0 && (unsigned)(ri + d0) <= (unsigned)A.nx && (unsigned)(rj + d1) <= (unsigned)A.ny &&
+                    (unsigned)(rk + d2) <= (unsigned)A.nz;
+    const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
     double Kv[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) Kv[q] = 0.0;
+    double du[3] = {0.0, 0.0, 0.0};
     if (nb) {
+      double w[kBoxMaxTerms];
+#pragma unroll
+      for (int e = 0; e < kBoxMaxTerms; ++e) w[e] = e < cnt ? W[hr + off[e]] : 0.0;
+      if constexpr (WITH_R) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) du[i] = ul[i * nh + hr + uoff] - ul[i * nh + hr];
+      }
 #pragma unroll
       for (int e = 0; e < kBoxMaxTerms; ++e)
 #pragma unroll
-        for (int q = 0; q < 9; ++q) Kv[q] = fma(G0.w[e], Bl[9 * e + q], Kv[q]);
+        for (int q = 0; q < 9; ++q) Kv[q] = fma(w[e], Bl[9 * e + q], Kv[q]);
     }
-    double S[9];
+    // group sums: the nine entries of the diagonal block (-sum of the row's other blocks)
+    // and the three residual components, each landing on one lane
+    double v12[12];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) S[q] = sum16(Kv[q]);
-    if (diag) {
+    for (int q = 0; q < 9; ++q) v12[q] = Kv[q];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) Kv[q] = -S[q];
+    for (int i = 0; i < 3; ++i) v12[9 + i] = Kv[3 * i] * du[0] + Kv[3 * i + 1] * du[1] + Kv[3 * i + 2] * du[2];
+    const double sum = reduce_scatter12(v12, lane);
+    const int idx = rs_index(lane);
+    if (live && idx >= 0 && idx < 9) {
+      const int dslot = (int)((pm >> (4 * kBoxDiagLane)) & 15);
+      seg[9 * (s0 - s_base + dslot) + idx] = -sum;
     }
     if constexpr (WITH_R) {
-      double du[3], ru[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) du[i] = nb ? G0.uc[i] - G0.ur[i] : 0.0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) ru[i] = sum16(Kv[3 * i] * du[0] + Kv[3 * i + 1] * du[1] + Kv[3 * i + 2] * du[2]);
-      if (live && lane < 3) {
+      if (live && idx >= 9) {
         const int k = A.lidx[row];
-        const double f = k >= 0 ? A.load_f[3 * (size_t)k + lane] : 0.0;
-        A.R[3 * (size_t)row + lane] = (lane == 0 ? ru[0] : (lane == 1 ? ru[1] : ru[2])) - f;
+        const double f = k >= 0 ? A.load_f[3 * (size_t)k + (idx - 9)] : 0.0;
+        A.R[3 * (size_t)row + (idx - 9)] = sum - f;
       }
     }
-    if (nb || diag) {
-      double* d = seg + 9 * (T0.s0 - T0.s_base + slot);
+    if (nb) {
+      double* d = seg + 9 * (s0 - s_base + slot);
 #pragma unroll
       for (int q = 0; q < 9; ++q) d[q] = Kv[q];
     }
     lds_barrier();
-    double* out = A.vals + 9 * (size_t)T0.s_base;
-    for (int q = threadIdx.x; q < 9 * (T0.s_end - T0.s_base); q += 256) __builtin_nontemporal_store(seg[q], out + q);
+    {  // 16-byte stores from the first 16-byte-aligned value on
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      double* out = A.vals + 9 * (size_t)s_base;
+      const int n = 9 * (s_end - s_base);
+      const int head = s_base & 1;  // 9 s_base odd: the segment starts 8 bytes past a 16-byte boundary
+      if (head && threadIdx.x == 0) __builtin_nontemporal_store(seg[0], out);
+      for (int q = head + 2 * (int)threadIdx.x; q + 1 < n; q += 512)
+        __builtin_nontemporal_store(v2d{seg[q], seg[q + 1]}, reinterpret_cast<v2d*>(out + q));
+      if (((n - head) & 1) && threadIdx.x == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
+    }
     lds_barrier();  // seg is rewritten by the next tile
-    T0 = T1;
-    G0 = G1;
-    T1 = T2;
-    has1 = has2;
   }
 }
 
