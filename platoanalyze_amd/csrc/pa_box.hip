@@ -50,40 +50,38 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
   }
 }
 
-// 16 lanes per graph row.  Lane l of every row computes the block of stencil neighbour
-// l (lane 15 the diagonal), so all rows read the same block table entries at the same
-// lane: the table reads are LDS broadcasts without bank conflicts, and a lane's element
-// offsets stay in registers.  Off-diagonal blocks sum at most six element terms,
-// gathered together; the diagonal block is -(sum of the row's other blocks) (every
-// element block row sums to zero: sum_b grad N_b = 0); R_r = sum_{c != r} K_rc (u_c -
-// u_r) - f_r; 16-lane sums by DPP.  The CRS slot of each lane's block comes from the
-// row's slot word.  A work-group's 16 consecutive rows are one contiguous CRS segment,
-// staged in LDS and streamed out with coalesced non-temporal stores.  Work-groups are
-// dealt to the XCDs in contiguous row ranges (round-robin dispatch puts group b on XCD
-// b % 8) and looped over within each XCD's range (a persistent grid: the block table
-// and the lane constants are loaded once per work-group). 
+// 8 lanes per graph row; lane j computes the blocks of stencil neighbours j and j + 8
+// (stencil lane 15 is the diagonal), so all rows read the same block table entries at
+// the same lane: the table reads are LDS broadcasts without bank conflicts, and a lane's
+// element offsets stay in registers.  An off-diagonal block sums at most six element
+// terms; the diagonal block is -(sum of the row's other blocks) (every element block row
+// sums to zero: sum_b grad N_b = 0); R_r = sum_{c != r} K_rc (u_c - u_r) - f_r.  The
+// nine diagonal entries and three residual components are summed over the row's lanes
+// in one DPP reduce-scatter.  The CRS slot of each block comes from the row's slot
+// word.  A work-group's 32 consecutive rows are one contiguous CRS segment, staged in LDS
+// and streamed out with coalesced non-temporal stores.  Work-groups are dealt to the
+// XCDs in contiguous row ranges (round-robin dispatch puts group b on XCD b % 8) and loop
+// over tiles within their XCD's range (a persistent grid: the block table and the lane
+// constants are loaded once per work-group).
+
 // Work-group barrier for LDS hand-offs only: waits for this wave's LDS operations, not
 // for its outstanding global stores.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// Sums of 12 values over each aligned 16-lane group, scattered over the lanes (a
-// reduce-scatter in four DPP exchanges): lane i receives the group sum of value index
-// rs_index(i), or nothing when rs_index(i) < 0.  Each exchange pairs every lane with a
-// partner holding the complementary half of the current value set.
-__device__ __forceinline__ int rs_index(int lane) {
-  const int o = ((lane & 2) ? 2 : 0) + (lane & 1);  // offset within the last three values
-  return o > 2 ? -1 : (lane < 8 ? 0 : 6) + ((lane & 7) < 4 ? 0 : 3) + o;
-}
-__device__ __forceinline__ double reduce_scatter12(const double v[12], int lane) {
-  const bool l1 = lane < 8, l2 = (lane & 7) < 4, l3 = (lane & 2) == 0, l4 = (lane & 1) == 0;
+// Sums of 12 values over each aligned 8-lane group, scattered: lane j (0..7) receives
+// the group sums of value indices rs_first(j) (in s0) and, for even j, rs_first(j) + 1
+// (in s1).  Three DPP exchanges, each pairing a lane with the partner holding the
+// complementary half of the current value set.
+__device__ __forceinline__ int rs_first(int j) { return (j < 4 ? 0 : 6) + ((j & 2) ? 3 : 0) + ((j & 1) ? 2 : 0); }
+__device__ __forceinline__ void reduce_scatter12x8(const double v[12], int j, double& s0, double& s1) {
+  const bool l1 = j < 4, l2 = (j & 2) == 0, l3 = (j & 1) == 0;
   double a[6], b[3];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) a[k] = (l1 ? v[k] : v[6 + k]) + dpp_f64<kDppMirror>(l1 ? v[6 + k] : v[k]);
+  for (int k = 0; k < 6; ++k) a[k] = (l1 ? v[k] : v[6 + k]) + dpp_f64<kDppHalfMirror>(l1 ? v[6 + k] : v[k]);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) b[k] = (l2 ? a[k] : a[3 + k]) + dpp_f64<kDppHalfMirror>(l2 ? a[3 + k] : a[k]);
-  const double c0 = (l3 ? b[0] : b[2]) + dpp_f64<kDppXor2>(l3 ? b[2] : b[0]);
-  const double c1 = (l3 ? b[1] : 0.0) + dpp_f64<kDppXor2>(l3 ? 0.0 : b[1]);
-  return (l4 ? c0 : c1) + dpp_f64<kDppXor1>(l4 ? c1 : c0);
+  for (int k = 0; k < 3; ++k) b[k] = (l2 ? a[k] : a[3 + k]) + dpp_f64<kDppXor2>(l2 ? a[3 + k] : a[k]);
+  s0 = (l3 ? b[0] : b[2]) + dpp_f64<kDppXor1>(l3 ? b[2] : b[0]);
+  s1 = (l3 ? b[1] : 0.0) + dpp_f64<kDppXor1>(l3 ? 0.0 : b[1]);
 }
 
 template <bool WITH_R>
@@ -91,20 +89,26 @@ __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
   __shared__ double sB[16 * kBoxMaxTerms * 9];
   __shared__ double seg[kBoxRows * kBoxMaxRow * 9];
   for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) sB[q] = A.B[q];
-  const int lane = threadIdx.x % 16, grp = threadIdx.x / 16;
-  // per-lane stencil constants, kept in registers for every tile
-  const int cnt = lane == kBoxDiagLane ? 0 : A.S->count[lane];
-  unsigned off[kBoxMaxTerms];
+  const int j = threadIdx.x % 8, grp = threadIdx.x / 8;
+  // constants of the lane's two stencil neighbours (j and j + 8), kept in registers
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  unsigned offa[kBoxMaxTerms], offb[kBoxMaxTerms];
 #pragma unroll
-  for (int e = 0; e < kBoxMaxTerms; ++e) off[e] = (unsigned)A.S->off[lane][e];
-  const int d0 = A.S->d[lane][0], d1 = A.S->d[lane][1], d2 = A.S->d[lane][2];
-  const unsigned uoff = (unsigned)A.S->uoff[lane];
+  for (int e = 0; e < kBoxMaxTerms; ++e) {
+    offa[e] = (unsigned)A.S->off[la][e];
+    offb[e] = (unsigned)A.S->off[lb][e];
+  }
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned uoffa = (unsigned)A.S->uoff[la], uoffb = (unsigned)A.S->uoff[lb];
   const unsigned px = A.nx + 2, py = A.ny + 2;
   const unsigned nh = px * py * (unsigned)(A.nz + 2);
   const double* __restrict__ W = A.W;
   const double* __restrict__ ul = A.ulat;
   __syncthreads();  // block table in LDS
-  const double* Bl = sB + lane * (kBoxMaxTerms * 9);
+  const double* Ba = sB + la * (kBoxMaxTerms * 9);
+  const double* Bb = sB + lb * (kBoxMaxTerms * 9);
   const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
   const int per = (nblk + kNumXcd - 1) / kNumXcd;
   const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
@@ -123,53 +127,81 @@ __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
       pm = A.perm[row];
     }
     const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
-    const int slot = (int)((pm >> (4 * lane)) & 15);
-    // this lane's stencil neighbour: present when inside the lattice (the host checked
-    // that every row holds exactly the present neighbours and itself)
-    const bool nb = live && cnt > 0 && (unsigned)(ri + d0) <= (unsigned)A.nx && (unsigned)(rj + d1) <= (unsigned)A.ny &&
-                    (unsigned)(rk + d2) <= (unsigned)A.nz;
+    // a stencil neighbour is present when inside the lattice (the host checked that
+    // every row holds exactly the present neighbours and itself)
+    const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+    const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
     const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
-    double Kv[9];
+    // Every index below stays inside the padded arrays for every lane: terms a lane does
+    // not have point at offset 0 with a zero block, ghost hexes weigh zero, and a lane
+    // without a neighbour reads the row's own u (du = 0).
+    double wa[kBoxMaxTerms], wb[kBoxMaxTerms];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) Kv[q] = 0.0;
-    double du[3] = {0.0, 0.0, 0.0};
-    if (nb) {
-      double w[kBoxMaxTerms];
-#pragma unroll
-      for (int e = 0; e < kBoxMaxTerms; ++e) w[e] = e < cnt ? W[hr + off[e]] : 0.0;
-      if constexpr (WITH_R) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) du[i] = ul[i * nh + hr + uoff] - ul[i * nh + hr];
-      }
-#pragma unroll
-      for (int e = 0; e < kBoxMaxTerms; ++e)
-#pragma unroll
-        for (int q = 0; q < 9; ++q) Kv[q] = fma(w[e], Bl[9 * e + q], Kv[q]);
+    for (int e = 0; e < kBoxMaxTerms; ++e) {
+      wa[e] = W[hr + offa[e]];
+      wb[e] = W[hr + offb[e]];
     }
-    // group sums: the nine entries of the diagonal block (-sum of the row's other blocks)
-    // and the three residual components, each landing on one lane
+    double dua[3] = {0.0, 0.0, 0.0}, dub[3] = {0.0, 0.0, 0.0};
+    if constexpr (WITH_R) {
+      const unsigned hca = nba ? hr + uoffa : hr, hcb = nbb ? hr + uoffb : hr;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double ur = ul[i * nh + hr];
+        dua[i] = ul[i * nh + hca] - ur;
+        dub[i] = ul[i * nh + hcb] - ur;
+      }
+    }
+    double Ka[9], Kb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      Ka[q] = wa[0] * Ba[q];
+      Kb[q] = wb[0] * Bb[q];
+    }
+#pragma unroll
+    for (int e = 1; e < kBoxMaxTerms; ++e)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        Ka[q] = fma(wa[e], Ba[9 * e + q], Ka[q]);
+        Kb[q] = fma(wb[e], Bb[9 * e + q], Kb[q]);
+      }
+    // row sums over the 8 lanes: the diagonal block (-sum of the other blocks) and R
     double v12[12];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) v12[q] = Kv[q];
+    for (int q = 0; q < 9; ++q) v12[q] = Ka[q] + Kb[q];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) v12[9 + i] = Kv[3 * i] * du[0] + Kv[3 * i + 1] * du[1] + Kv[3 * i + 2] * du[2];
-    const double sum = reduce_scatter12(v12, lane);
-    const int idx = rs_index(lane);
-    if (live && idx >= 0 && idx < 9) {
-      const int dslot = (int)((pm >> (4 * kBoxDiagLane)) & 15);
-      seg[9 * (s0 - s_base + dslot) + idx] = -sum;
-    }
-    if constexpr (WITH_R) {
-      if (live && idx >= 9) {
+    for (int i = 0; i < 3; ++i)
+      v12[9 + i] = Ka[3 * i] * dua[0] + Ka[3 * i + 1] * dua[1] + Ka[3 * i + 2] * dua[2] +
+                   Kb[3 * i] * dub[0] + Kb[3 * i + 1] * dub[1] + Kb[3 * i + 2] * dub[2];
+    double sum0, sum1;
+    reduce_scatter12x8(v12, j, sum0, sum1);
+    if (live) {
+      const int i0 = rs_first(j);
+      double* dg = seg + 9 * (s0 - s_base + (int)((pm >> (4 * kBoxDiagLane)) & 15));
+      if (i0 < 9) dg[i0] = -sum0;
+      if ((j & 1) == 0 && i0 + 1 < 9) dg[i0 + 1] = -sum1;
+      if constexpr (WITH_R) {
         const int k = A.lidx[row];
-        const double f = k >= 0 ? A.load_f[3 * (size_t)k + (idx - 9)] : 0.0;
-        A.R[3 * (size_t)row + (idx - 9)] = sum - f;
+        if (i0 >= 9) {
+          const double f = k >= 0 ? A.load_f[3 * (size_t)k + (i0 - 9)] : 0.0;
+          A.R[3 * (size_t)row + (i0 - 9)] = sum0 - f;
+        }
+        if ((j & 1) == 0 && i0 + 1 >= 9) {
+          const double f = k >= 0 ? A.load_f[3 * (size_t)k + (i0 - 8)] : 0.0;
+          A.R[3 * (size_t)row + (i0 - 8)] = sum1 - f;
+        }
       }
     }
-    if (nb) {
-      double* d = seg + 9 * (s0 - s_base + slot);
+    if (nba) {
+      double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * la)) & 15));
 #pragma unroll
-      for (int q = 0; q < 9; ++q) d[q] = Kv[q];
+      for (int q = 0; q < 9; ++q) d[q] = Ka[q];
+    }
+    if (nbb) {
+      double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * lb)) & 15));
+#pragma unroll
+      for (int q = 0; q < 9; ++q) d[q] = Kb[q];
     }
     lds_barrier();
     {  // 16-byte stores from the first 16-byte-aligned value on

@@ -17,8 +17,8 @@
 
 namespace pa {
 
-constexpr int kBoxMaxRow = 16;     // graph row length limit (one 16-lane group per row)
-constexpr int kBoxRows = 16;       // rows per 256-thread work-group
+constexpr int kBoxMaxRow = 16;     // graph row length limit (16 stencil lanes per row)
+constexpr int kBoxRows = 32;       // rows per 256-thread work-group tile (8 lanes per row)
 constexpr int kBoxMaxTerms = 6;    // element terms per off-diagonal block
 constexpr int kBoxDiagLane = 15;   // stencil lane of the diagonal block
 constexpr int kBoxMaxAxis = 1024;  // nodes per axis (10-bit packed lattice positions)
@@ -66,7 +66,7 @@ __global__ void k_box_w(BoxArgs A);
 // B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T.
 __global__ void k_box_material(const double* P, double* B, double lam, double mu);
 
-// K values of every graph row (and R = K u - f when WITH_R); one work-group per 16 rows.
+// K values of every graph row (and R = K u - f when WITH_R); 32-row tiles per work-group.
 template <bool WITH_R>
 __global__ void k_box_jac(BoxArgs A);
 
