@@ -28,6 +28,7 @@ __global__ __launch_bounds__(256) void k_box_scatter(BoxArgs A) {
   }
 }
 
+template <bool SIMP_INT>
 __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
   const int px = A.nx + 2, py = A.ny + 2, pz = A.nz + 2;
   const size_t nh = (size_t)px * py * pz;
@@ -45,10 +46,14 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
     for (int t = 0; t < 6; ++t) {
       const uint8_t* cr = A.corners + 4 * t;
       const double rho = (zc[cr[0]] + zc[cr[1]] + zc[cr[2]] + zc[cr[3]]) / 4.0;
-      A.W[t * nh + hp] = penalty_w(A.m, rho);
+      // SIMP with a small integer exponent: penalty_w's own multiplication path, inlined
+      // without the other models' code
+      A.W[t * nh + hp] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
     }
   }
 }
+template __global__ void k_box_w<true>(BoxArgs);
+template __global__ void k_box_w<false>(BoxArgs);
 
 // 8 lanes per graph row; lane j computes the blocks of stencil neighbours j and j + 8
 // (stencil lane 15 is the diagonal), so all rows read the same block table entries at

@@ -61,6 +61,8 @@ __global__ void k_box_scatter(BoxArgs A);
 
 // W for every padded hex from the lattice-ordered z (one thread per hex; ghost hexes get
 // zero weights).
+// SIMP_INT: the material's penalty is SIMP with an integer exponent (Material::pint > 0).
+template <bool SIMP_INT>
 __global__ void k_box_w(BoxArgs A);
 
 // B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T.

@@ -1210,7 +1210,10 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
   }
   const size_t nh = (size_t)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
   pa::k_box_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, h->stream>>>(a);
-  pa::k_box_w<<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
+  if (a.m.model == pa::kPenaltySimp && a.m.pint > 0)
+    pa::k_box_w<true><<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
+  else
+    pa::k_box_w<false><<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
   const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
   const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, R ? h->box_grid_r : h->box_grid_k);
   if (R)
