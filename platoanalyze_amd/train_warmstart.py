@@ -3,10 +3,12 @@
     python -m platoanalyze_amd.train_warmstart --config cantilever_64x32x32 \
         --traj 20 --iters 100 --epochs 10 --out weights.npz
 
-Data: one FP64 solve per OC iteration (random volume fraction and traction direction
-per trajectory); the last trajectory is held out.  Evaluation: on the held-out designs,
-the mixed-precision solve to 1e-8 from zero against the same solve from the network's
-warm start (iterations and seconds), both judged by the true FP64 residual.
+Data: one multigrid solve per OC iteration (random volume fraction and traction
+direction per trajectory); the last trajectory is held out.  Evaluation: on the held-out
+designs, the mixed-precision solve to 1e-8 from zero against the same solve from the
+network's warm start (iterations and seconds), both judged by the true FP64 residual;
+in update mode also from the previous design's solution, with Jacobi and multigrid
+preconditioning.
 """
 from __future__ import annotations
 
@@ -46,6 +48,34 @@ def evaluate(net, c0, held, tol=1e-8, log=print, mode=None):
     return rows
 
 
+def evaluate_update(net, c0, held, tol=1e-8, log=print, preconditioner="jacobi"):
+    """Update mode: each held-out design solved from zero, from the previous design's
+    solution and from the network's update of it (iterations, energy excess)."""
+    from . import workloads as wl
+    from .elliptic import Problem
+    from .warmstart import WarmStart, energy
+
+    rows = []
+    for w, zp, up, z, f in held:
+        m = wl.model(w)
+        ws = WarmStart(net, w, m, c0, mode="update")
+        pb = Problem(m, tolerance=tol, check_every=4 if preconditioner == "multigrid" else 16, precision="mixed",
+                     preconditioner=preconditioner)
+        u = pb.solution(z).clone()
+        it_cold = pb.stats.iterations
+        pb.solution(z, u0=up)
+        it_prev = pb.stats.iterations
+        u0 = ws.initial_guess(z, f, zp, up)
+        u_w = pb.solution(z, u0=u0)
+        it_warm = pb.stats.iterations
+        e_star = energy(m, z, u, f)
+        rows.append({"iters_cold": it_cold, "iters_prev": it_prev, "iters_warm": it_warm,
+                     "energy_excess_prev": energy(m, z, up, f) - e_star, "energy_excess_warm": energy(m, z, u0, f) - e_star,
+                     "u_rel_diff": float((u_w - u).norm() / u.norm())})
+        log(json.dumps(rows[-1]))
+    return rows
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cantilever_64x32x32")
@@ -57,7 +87,7 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--channels", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--mode", default="load", choices=["load", "shape", "basis"])
+    ap.add_argument("--mode", default="load", choices=["load", "shape", "basis", "update"])
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
 
@@ -71,15 +101,23 @@ def main(argv=None) -> int:
     net, c0, hist, held = train(base, a.traj, a.iters, a.epochs, batch=a.batch, lr=a.lr, base_channels=a.channels,
                             seed=a.seed, mode=a.mode, log=log)
     log(f"training done in {time.perf_counter() - t0:.1f} s")
-    idx = np.linspace(0, len(held) - 1, min(5, len(held))).astype(int)
-    rows = evaluate(net, c0, [held[i] for i in idx], log=log, mode=a.mode)
+    idx = np.linspace(0, len(held) - 1, min(8, len(held))).astype(int)
+    if a.mode == "update":
+        rows = []
+        for pre in ("jacobi", "multigrid"):
+            log(f"evaluation, {pre} preconditioner")
+            rows += evaluate_update(net, c0, [held[i] for i in idx], log=log, preconditioner=pre)
+    else:
+        rows = evaluate(net, c0, [held[i] for i in idx], log=log, mode=a.mode)
     meta = {"config": base.cfg.name, "mode": a.mode, "traj": a.traj, "iters": a.iters, "epochs": a.epochs,
             "loss_history": np.array(hist), "eval_iters_cold": np.array([r["iters_cold"] for r in rows]),
             "eval_iters_warm": np.array([r["iters_warm"] for r in rows])}
     save(net, c0, a.out, meta)
     log(json.dumps({"saved": a.out, "final_loss": hist[-1] if hist else None,
                     "mean_iters_cold": float(np.mean([r["iters_cold"] for r in rows])),
-                    "mean_iters_warm": float(np.mean([r["iters_warm"] for r in rows]))}))
+                    "mean_iters_warm": float(np.mean([r["iters_warm"] for r in rows])),
+                    "mean_iters_prev": (float(np.mean([r["iters_prev"] for r in rows])) if a.mode == "update"
+                                        else None)}))
     return 0
 
 

@@ -74,6 +74,21 @@ class Features:
         x[self.vox, 7] = (self.rmin + (1.0 - self.rmin) * z.pow(self.p)).to(torch.float32)  # SIMP stiffness
         return x.reshape(*self.dims, IN_CH)
 
+    def inputs_update(self, z: torch.Tensor, z_prev: torch.Tensor, u_prev: torch.Tensor, f: torch.Tensor,
+                      cu: float) -> torch.Tensor:
+        """Update mode: the design, its change since the previous design, the previous
+        displacement (load-scaled, times cu), the SIMP stiffness and its change, x."""
+        x = torch.zeros((self.V, IN_CH), dtype=torch.float32, device=z.device)
+        wz = self.rmin + (1.0 - self.rmin) * z.pow(self.p)
+        wp = self.rmin + (1.0 - self.rmin) * z_prev.pow(self.p)
+        x[self.vox, 0] = z.to(torch.float32)
+        x[self.vox, 1] = (5.0 * (z - z_prev)).to(torch.float32)
+        x[self.vox, 2:5] = (u_prev * (cu / self.load_scale(f))).reshape(-1, 3).to(torch.float32)
+        x[self.vox, 5] = wz.to(torch.float32)
+        x[self.vox, 6] = (5.0 * (wz - wp)).to(torch.float32)
+        x[self.vox, 7] = self.xyz[:, 0]
+        return x.reshape(*self.dims, IN_CH)
+
     @staticmethod
     def load_scale(f: torch.Tensor) -> float:
         return float(f.reshape(-1, 3).abs().max().clamp_min(1e-300))
@@ -117,6 +132,26 @@ def galerkin_scale(model, z: torch.Tensor, u_hat: torch.Tensor, f: torch.Tensor,
     return num / den if den > 0 else 0.0
 
 
+def line_search(model, z: torch.Tensor, u: torch.Tensor, d: torch.Tensor, f: torch.Tensor) -> float:
+    """alpha minimising the energy of u + alpha d: d.(f - K u) / d.K d (0 if d.K d <= 0);
+    d must vanish on the constrained dofs."""
+    from .elliptic import VectorFunction
+
+    vf = VectorFunction(model)
+    r = vf.value(u, z)  # K u - f
+    Kd = vf.value(d, z) + f
+    den = float(torch.dot(d, Kd))
+    return -float(torch.dot(d, r)) / den if den > 0 else 0.0
+
+
+def energy(model, z: torch.Tensor, u: torch.Tensor, f: torch.Tensor) -> float:
+    """1/2 u.K u - f.u (its excess over the solution's is half the squared energy-norm error)."""
+    from .elliptic import VectorFunction
+
+    Ku = VectorFunction(model).value(u, z) + f
+    return 0.5 * float(torch.dot(u, Ku)) - float(torch.dot(f, u))
+
+
 def smooth(model, z: torch.Tensor, u: torch.Tensor, steps: int, omega: float = 0.6) -> None:
     """In place: `steps` damped block-Jacobi sweeps of the homogeneous problem (pa_smooth)."""
     model._stream()
@@ -130,13 +165,22 @@ class WarmStart:
     factor (c0 <= 0);
     mode "basis": it predicts, from the design alone, the responses to unit tractions
     along x, y and z on the loaded face, combined with the problem's traction vector
-    outside the network (u is linear in the load, exactly)."""
+    outside the network (u is linear in the load, exactly);
+    mode "update" (an optimisation loop): from the previous design and its solution it
+    predicts the change of the displacement; the guess is u_prev + alpha d with the
+    energy-optimal alpha along the prediction d, so it is never worse in the energy norm
+    than starting from u_prev (c0 = (cu, cd): the input and output scales)."""
 
     def __init__(self, net: UNet, w: wl.Workload, model, c0: float, smooth_steps: int = 8,
                  mode: str | None = None):
         self.net = net
         self.smooth_steps = int(smooth_steps)
-        self.c0 = float(c0)
+        if isinstance(c0, (tuple, list)):
+            self.cu, self.cd = float(c0[0]), float(c0[1])
+            self.c0 = self.cd
+            mode = mode or "update"
+        else:
+            self.c0 = float(c0)
         self.mode = mode or ("load" if self.c0 > 0 else "shape")
         self.w = w
         self.model = model
@@ -145,9 +189,17 @@ class WarmStart:
             raise ValueError(f"network grid {net.grid} != mesh node grid {self.feat.dims}")
         self.bc = torch.as_tensor(w.bc_dofs, dtype=torch.long, device="cuda")
 
-    def initial_guess(self, z: torch.Tensor, f: torch.Tensor | None = None) -> torch.Tensor:
+    def initial_guess(self, z: torch.Tensor, f: torch.Tensor | None = None, z_prev: torch.Tensor | None = None,
+                      u_prev: torch.Tensor | None = None) -> torch.Tensor:
         if f is None:
             f = load_vector(self.model, z)
+        if self.mode == "update":
+            if z_prev is None or u_prev is None:
+                raise ValueError("update mode needs the previous design and its solution")
+            y = self.net.forward(self.feat.inputs_update(z, z_prev, u_prev, f, self.cu).unsqueeze(0).contiguous())
+            d = self.feat.displacement(y, f, self.cd)
+            d[self.bc] = 0.0
+            return u_prev + line_search(self.model, z, u_prev, d, f) * d
         if self.mode == "basis":
             t = np.asarray(self.w.traction, dtype=np.float64)
             fs = self.feat.load_scale(f) / max(float(np.abs(t).max()), 1e-300)  # unit-traction load scale
@@ -174,14 +226,15 @@ def save(net: UNet, c0: float, path: str | Path, meta: dict) -> None:
              **{k: np.array(v) for k, v in meta.items()})
 
 
-def load(path: str | Path, grid: tuple | None = None) -> tuple[UNet, float]:
-    """Load (network, c0); `grid` re-instantiates the (fully convolutional) network on
-    another node grid."""
+def load(path: str | Path, grid: tuple | None = None):
+    """Load (network, c0) -- c0 a pair (cu, cd) in update mode; `grid` re-instantiates the
+    (fully convolutional) network on another node grid."""
     d = np.load(path, allow_pickle=False)
     g = tuple(int(v) for v in (grid if grid is not None else d["grid"]))
     net = UNet(g, int(d["base"]))
     net.set_params(torch.as_tensor(d["params"].astype(np.float32)))
-    return net, float(d["c0"])
+    c0 = np.asarray(d["c0"], dtype=np.float64)
+    return net, (tuple(float(v) for v in c0) if c0.size == 2 else float(c0))
 
 
 # ------------------------------------------------------------------ training data
@@ -222,14 +275,14 @@ def trajectories(base: wl.Workload, n_traj: int, n_iter: int, seed: int = 7, tol
         d /= np.linalg.norm(d)
         w = dataclasses.replace(base, traction=tuple(float(v) for v in d))
         m = wl.model(w)
-        pb = Problem(m, tolerance=tol, check_every=32)
+        pb = Problem(m, tolerance=tol, check_every=4, precision="mixed", preconditioner="multigrid")
         z = np.full(base.mesh.num_nodes, vf)
         zt = torch.as_tensor(z, device="cuda")
         f = load_vector(m, zt)
         t0 = time.perf_counter()
         for k in range(n_iter):
             zt = torch.as_tensor(z, device="cuda")
-            u = pb.solution(zt)
+            u = pb.solution(zt).clone()
             g, Cv = pb.criterionGradient(zt, u)
             yield t, k, w, m, zt, f, u
             z = oc_update(z, g.cpu().numpy(), wn, vf, base.cfg.rho_min)
@@ -304,12 +357,54 @@ def train_basis(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch:
     return net, c0, hist, held
 
 
+def train_update(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch: int = 4, lr: float = 1e-3,
+                 base_channels: int = 16, seed: int = 0, holdout: int = 1, log=print):
+    """Update mode (see WarmStart): pairs of consecutive OC designs; returns (net, (cu,
+    cd), history, held-out [(w, z_prev, u_prev, z, f)])."""
+    feat = Features(base)
+    pairs, held = [], []
+    prev = {}
+    for t, k, w, m, z, f, u in trajectories(base, n_traj + holdout, n_iter, seed=seed + 11, log=log):
+        if k > 0:
+            zp, up = prev[t]
+            if t < n_traj:
+                pairs.append((zp, up, z.clone(), f.clone(), u))
+            else:
+                held.append((w, zp, up, z.clone(), f.clone()))
+        prev[t] = (z.clone(), u)
+    fs = [feat.load_scale(f) for _, _, _, f, _ in pairs]
+    cu = float(1.0 / torch.stack([(up / s).pow(2).mean() for (_, up, _, _, _), s in zip(pairs, fs)]).mean().sqrt())
+    cd = float(1.0 / torch.stack([((u - up) / s).pow(2).mean() for (_, up, _, _, u), s in zip(pairs, fs)])
+               .mean().sqrt())
+    X = torch.stack([feat.inputs_update(z, zp, up, f, cu) for zp, up, z, f, _ in pairs])
+    Y = torch.stack([feat.target(u - up, f, cd) for _, up, _, f, u in pairs])
+    del pairs
+    net = UNet(feat.dims, base_channels, seed=seed)
+    rng = np.random.default_rng(seed)
+    hist = []
+    n = X.shape[0]
+    for ep in range(epochs):
+        lr_ep = lr * 0.5 * (1.0 + np.cos(np.pi * ep / max(epochs, 1)))
+        perm = rng.permutation(n)
+        tot, cnt = 0.0, 0
+        t0 = time.perf_counter()
+        for i in range(0, n - batch + 1, batch):
+            idx = torch.as_tensor(perm[i:i + batch], device="cuda")
+            tot += net.train_step(X[idx].contiguous(), Y[idx].contiguous(), lr_ep)
+            cnt += 1
+        hist.append(tot / max(cnt, 1))
+        log(f"epoch {ep}: loss {hist[-1]:.4e} ({time.perf_counter() - t0:.1f} s, {cnt} steps)")
+    return net, (cu, cd), hist, held
+
+
 def train(base: wl.Workload, n_traj: int, n_iter: int, epochs: int, batch: int = 4, lr: float = 1e-3,
           base_channels: int = 16, seed: int = 0, holdout: int = 1, mode: str = "load", log=print):
     """Generate data along trajectories, train, and return (net, c0, history, held-out
     set); c0 <= 0 marks the "shape" mode (see WarmStart)."""
     if mode == "basis":
         return train_basis(base, n_traj, n_iter, epochs, batch, lr, base_channels, seed, holdout, log)
+    if mode == "update":
+        return train_update(base, n_traj, n_iter, epochs, batch, lr, base_channels, seed, holdout, log)
     feat = Features(base)
     X, U, held = [], [], []
     for t, k, w, m, z, f, u in trajectories(base, n_traj + holdout, n_iter, seed=seed + 11, log=log):
