@@ -64,7 +64,7 @@ template __global__ void k_box_w<false>(BoxArgs);
 // nine diagonal entries and three residual components are summed over the row's lanes
 // in one DPP reduce-scatter.  The CRS slot of each block comes from the row's slot
 // word.  A work-group's 32 consecutive rows are one contiguous CRS segment, staged in LDS
-// and streamed out with coalesced non-temporal stores.  Work-groups are dealt to the
+// in two halves of 16 rows and streamed out with coalesced non-temporal stores.  Work-groups are dealt to the
 // XCDs in contiguous row ranges (round-robin dispatch puts group b on XCD b % 8) and loop
 // over tiles within their XCD's range (a persistent grid: the block table and the lane
 // constants are loaded once per work-group).
@@ -90,9 +90,9 @@ __device__ __forceinline__ void reduce_scatter12x8(const double v[12], int j, do
 }
 
 template <bool WITH_R>
-__global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
+__global__ __launch_bounds__(256, 4) void k_box_jac(BoxArgs A) {
   __shared__ double sB[16 * kBoxMaxTerms * 9];
-  __shared__ double seg[kBoxRows * kBoxMaxRow * 9];
+  __shared__ double seg[(kBoxRows / 2) * kBoxMaxRow * 9];  // one half-tile of 16 rows
   for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) sB[q] = A.B[q];
   const int j = threadIdx.x % 8, grp = threadIdx.x / 8;
   // constants of the lane's two stencil neighbours (j and j + 8), kept in registers
@@ -183,9 +183,6 @@ __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
     reduce_scatter12x8(v12, j, sum0, sum1);
     if (live) {
       const int i0 = rs_first(j);
-      double* dg = seg + 9 * (s0 - s_base + (int)((pm >> (4 * kBoxDiagLane)) & 15));
-      if (i0 < 9) dg[i0] = -sum0;
-      if ((j & 1) == 0 && i0 + 1 < 9) dg[i0 + 1] = -sum1;
       if constexpr (WITH_R) {
         const int k = A.lidx[row];
         if (i0 >= 9) {
@@ -198,28 +195,43 @@ __global__ __launch_bounds__(256) void k_box_jac(BoxArgs A) {
         }
       }
     }
-    if (nba) {
-      double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * la)) & 15));
+    // the tile's two halves of 16 rows are staged and streamed out in turn
+    const int mid = min(base + kBoxRows / 2, last);
+    const int s_mid = A.rowmap[mid];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) d[q] = Ka[q];
-    }
-    if (nbb) {
-      double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * lb)) & 15));
+    for (int hf = 0; hf < 2; ++hf) {
+      const int sb = hf ? s_mid : s_base, se = hf ? s_end : s_mid;
+      if ((grp >> 4) == hf) {
+        if (live) {
+          const int i0 = rs_first(j);
+          double* dg = seg + 9 * (s0 - sb + (int)((pm >> (4 * kBoxDiagLane)) & 15));
+          if (i0 < 9) dg[i0] = -sum0;
+          if ((j & 1) == 0 && i0 + 1 < 9) dg[i0 + 1] = -sum1;
+        }
+        if (nba) {
+          double* d = seg + 9 * (s0 - sb + (int)((pm >> (4 * la)) & 15));
 #pragma unroll
-      for (int q = 0; q < 9; ++q) d[q] = Kb[q];
+          for (int q = 0; q < 9; ++q) d[q] = Ka[q];
+        }
+        if (nbb) {
+          double* d = seg + 9 * (s0 - sb + (int)((pm >> (4 * lb)) & 15));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) d[q] = Kb[q];
+        }
+      }
+      lds_barrier();
+      {  // 16-byte stores from the first 16-byte-aligned value on
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        double* out = A.vals + 9 * (size_t)sb;
+        const int n = 9 * (se - sb);
+        const int head = sb & 1;  // 9 sb odd: the segment starts 8 bytes past a 16-byte boundary
+        if (head && threadIdx.x == 0 && n > 0) __builtin_nontemporal_store(seg[0], out);
+        for (int q = head + 2 * (int)threadIdx.x; q + 1 < n; q += 512)
+          __builtin_nontemporal_store(v2d{seg[q], seg[q + 1]}, reinterpret_cast<v2d*>(out + q));
+        if (n > head && ((n - head) & 1) && threadIdx.x == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
+      }
+      lds_barrier();  // seg is rewritten by the next half
     }
-    lds_barrier();
-    {  // 16-byte stores from the first 16-byte-aligned value on
-      typedef double v2d __attribute__((ext_vector_type(2)));
-      double* out = A.vals + 9 * (size_t)s_base;
-      const int n = 9 * (s_end - s_base);
-      const int head = s_base & 1;  // 9 s_base odd: the segment starts 8 bytes past a 16-byte boundary
-      if (head && threadIdx.x == 0) __builtin_nontemporal_store(seg[0], out);
-      for (int q = head + 2 * (int)threadIdx.x; q + 1 < n; q += 512)
-        __builtin_nontemporal_store(v2d{seg[q], seg[q + 1]}, reinterpret_cast<v2d*>(out + q));
-      if (((n - head) & 1) && threadIdx.x == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
-    }
-    lds_barrier();  // seg is rewritten by the next tile
   }
 }
 
