@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
     const int I = (int)(hp % px), J = (int)((hp / px) % py), K = (int)(hp / ((size_t)px * py));
     if (I < 1 || J < 1 || K < 1 || I > A.nx || J > A.ny || K > A.nz) {  // ghost hex
 #pragma unroll
-      for (int t = 0; t < 6; ++t) A.W[t * nh + hp] = 0.0;
+      for (int t = 0; t < 6; ++t) A.W[6 * hp + t] = 0.0;
       continue;
     }
     double zc[8];  // corners of the hex whose lower corner is lattice point (I-1, J-1, K-1)
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
       const double rho = (zc[cr[0]] + zc[cr[1]] + zc[cr[2]] + zc[cr[3]]) / 4.0;
       // SIMP with a small integer exponent: penalty_w's own multiplication path, inlined
       // without the other models' code
-      A.W[t * nh + hp] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
+      A.W[6 * hp + t] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
     }
   }
 }
@@ -145,8 +145,8 @@ __global__ __launch_bounds__(256, 4) void k_box_jac(BoxArgs A) {
     double wa[kBoxMaxTerms], wb[kBoxMaxTerms];
 #pragma unroll
     for (int e = 0; e < kBoxMaxTerms; ++e) {
-      wa[e] = W[hr + offa[e]];
-      wb[e] = W[hr + offb[e]];
+      wa[e] = W[6 * hr + offa[e]];
+      wb[e] = W[6 * hr + offb[e]];
     }
     double dua[3] = {0.0, 0.0, 0.0}, dub[3] = {0.0, 0.0, 0.0};
     if constexpr (WITH_R) {

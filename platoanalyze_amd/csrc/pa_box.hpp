@@ -24,8 +24,8 @@ constexpr int kBoxDiagLane = 15;   // stencil lane of the diagonal block
 constexpr int kBoxMaxAxis = 1024;  // nodes per axis (10-bit packed lattice positions)
 
 // Per stencil neighbour l (< 15; lane 15 is the diagonal): its lattice offset, the
-// number of element terms and, per term e, the element-weight offset from the row's
-// padded hex index (t * nhex - hex offset) and the unit-weight block P of (t, a, b).
+// number of element terms and, per term e, the element-weight offset from six times the
+// row's padded hex index (6 * (-hex offset) + t) and the unit-weight block P of (t, a, b).
 struct BoxStencil {
   int count[16];
   int off[16][kBoxMaxTerms];
@@ -40,7 +40,7 @@ struct BoxArgs {
   const unsigned long long* perm;  // per row: CRS slot of stencil lane l in bits 4l..4l+3
   const uint8_t* corners;  // [6][4] corner of each local vertex of the lattice tets (bit 0 x, 1 y, 2 z)
   const double* z;
-  double* W;               // [6][(nx+2)(ny+2)(nz+2)] element weights, zero on the ghost layer
+  double* W;               // [(nx+2)(ny+2)(nz+2)][6] element weights per padded hex, zero on the ghost layer
   const double* B;         // [16][6][9] material blocks per stencil lane and term at unit weight
   const BoxStencil* S;
   const double* u;

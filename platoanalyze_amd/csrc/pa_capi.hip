@@ -1072,7 +1072,7 @@ static int box_fast_setup(pa_handle h) {
   // hc of the hex whose lower corner is its lattice point minus the bits of hc; in tet t
   // of that hex it is local vertex a, the column node local vertex b.
   const size_t nhex = (size_t)(hx + 2) * (hy + 2) * (hz + 2);
-  if (6 * nhex >= (size_t)1 << 31) return PA_OK;
+  if (6 * nhex >= (size_t)1 << 32) return PA_OK;  // 32-bit element-weight offsets
   const int px = hx + 2, py = hy + 2;
   std::vector<std::vector<std::array<int, 4>>> terms(27);  // (hc, t, a, b) per offset
   for (int hc = 0; hc < 8; ++hc)
@@ -1102,7 +1102,7 @@ static int box_fast_setup(pa_handle h) {
     for (size_t e = 0; e < terms[di].size(); ++e) {
       const auto& tm = terms[di][e];
       const int hc = tm[0], t = tm[1], a = tm[2], b = tm[3];
-      st.off[l][e] = (int)(t * nhex) - ((hc & 1) + px * ((hc >> 1) & 1) + px * py * (hc >> 2));
+      st.off[l][e] = t - 6 * ((hc & 1) + px * ((hc >> 1) & 1) + px * py * (hc >> 2));
       for (int q = 0; q < 9; ++q) P[9 * (l * pa::kBoxMaxTerms + e) + q] = Pt[9 * ((t * 4 + a) * 4 + b) + q];
     }
   }
