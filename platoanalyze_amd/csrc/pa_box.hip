@@ -34,22 +34,27 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
   const size_t nh = (size_t)px * py * pz;
   for (size_t hp = blockIdx.x * (size_t)256 + threadIdx.x; hp < nh; hp += (size_t)gridDim.x * 256) {
     const int I = (int)(hp % px), J = (int)((hp / px) % py), K = (int)(hp / ((size_t)px * py));
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    v2d* w2 = reinterpret_cast<v2d*>(A.W + 6 * hp);  // 48-byte records: three 16-byte stores
     if (I < 1 || J < 1 || K < 1 || I > A.nx || J > A.ny || K > A.nz) {  // ghost hex
 #pragma unroll
-      for (int t = 0; t < 6; ++t) A.W[6 * hp + t] = 0.0;
+      for (int k = 0; k < 3; ++k) w2[k] = v2d{0.0, 0.0};
       continue;
     }
     double zc[8];  // corners of the hex whose lower corner is lattice point (I-1, J-1, K-1)
 #pragma unroll
     for (int c = 0; c < 8; ++c) zc[c] = A.zlat[hp + (c & 1) + px * (((c >> 1) & 1) + (size_t)py * (c >> 2))];
+    double w6[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
       const uint8_t* cr = A.corners + 4 * t;
       const double rho = (zc[cr[0]] + zc[cr[1]] + zc[cr[2]] + zc[cr[3]]) / 4.0;
       // SIMP with a small integer exponent: penalty_w's own multiplication path, inlined
       // without the other models' code
-      A.W[6 * hp + t] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
+      w6[t] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) w2[k] = v2d{w6[2 * k], w6[2 * k + 1]};
   }
 }
 template __global__ void k_box_w<true>(BoxArgs);
