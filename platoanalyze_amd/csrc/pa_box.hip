@@ -23,7 +23,7 @@ __global__ __launch_bounds__(256) void k_box_scatter(BoxArgs A) {
     A.zlat[hp] = A.z[n];
     if (A.ulat) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) A.ulat[c * nh + hp] = A.u[3 * (size_t)n + c];
+      for (int c = 0; c < 3; ++c) A.ulat[3 * hp + c] = A.u[3 * (size_t)n + c];
     }
   }
 }
@@ -158,9 +158,9 @@ __global__ __launch_bounds__(256, 4) void k_box_jac(BoxArgs A) {
       const unsigned hca = nba ? hr + uoffa : hr, hcb = nbb ? hr + uoffb : hr;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const double ur = ul[i * nh + hr];
-        dua[i] = ul[i * nh + hca] - ur;
-        dub[i] = ul[i * nh + hcb] - ur;
+        const double ur = ul[3 * hr + i];
+        dua[i] = ul[3 * hca + i] - ur;
+        dub[i] = ul[3 * hcb + i] - ur;
       }
     }
     double Ka[9], Kb[9];

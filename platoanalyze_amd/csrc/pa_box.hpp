@@ -44,7 +44,7 @@ struct BoxArgs {
   const double* B;         // [16][6][9] material blocks per stencil lane and term at unit weight
   const BoxStencil* S;
   const double* u;
-  double* ulat;            // [3][padded lattice] u at node lattice points (padded like the hexes)
+  double* ulat;            // [padded lattice][3] u at node lattice points (padded like the hexes)
   double* zlat;            // [padded lattice] z at node lattice points
   const int* lidx;         // per node: its entry in the load list, or -1
   const double* load_f;    // [3 * loads] nodal loads f
