@@ -30,25 +30,35 @@ __global__ __launch_bounds__(256) void k_box_scatter(BoxArgs A) {
 
 template <bool SIMP_INT>
 __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
-  const int px = A.nx + 2, py = A.ny + 2, pz = A.nz + 2;
-  const size_t nh = (size_t)px * py * pz;
-  for (size_t hp = blockIdx.x * (size_t)256 + threadIdx.x; hp < nh; hp += (size_t)gridDim.x * 256) {
-    const int I = (int)(hp % px), J = (int)((hp / px) % py), K = (int)(hp / ((size_t)px * py));
+  // One wave per run of 64 padded hexes along x: the run's line (J, K) and start are
+  // wave-uniform, so the index arithmetic is 32-bit scalar work (6 nh < 2^32 was checked
+  // at set-up).
+  const unsigned px = A.nx + 2, py = A.ny + 2, pz = A.nz + 2;
+  const unsigned nseg = (px + 63) / 64, nwave = nseg * py * pz;
+  const unsigned lane = threadIdx.x % 64;
+  const unsigned w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + threadIdx.x / 64);
+  int cr[24];  // corner of each local vertex of the six tets, wave-uniform (scalar registers)
+#pragma unroll
+  for (int q = 0; q < 24; ++q) cr[q] = __builtin_amdgcn_readfirstlane((int)A.corners[q]);
+  for (unsigned w = w0; w < nwave; w += gridDim.x * 4) {
+    const unsigned line = w / nseg, I = (w - line * nseg) * 64 + lane;
+    const unsigned J = line % py, K = line / py;
+    if (I >= px) continue;
+    const unsigned hp = I + px * line;
     typedef double v2d __attribute__((ext_vector_type(2)));
-    v2d* w2 = reinterpret_cast<v2d*>(A.W + 6 * hp);  // 48-byte records: three 16-byte stores
-    if (I < 1 || J < 1 || K < 1 || I > A.nx || J > A.ny || K > A.nz) {  // ghost hex
+    v2d* w2 = reinterpret_cast<v2d*>(A.W + 6 * (size_t)hp);  // 48-byte records: three 16-byte stores
+    if (I < 1 || J < 1 || K < 1 || I > (unsigned)A.nx || J > (unsigned)A.ny || K > (unsigned)A.nz) {  // ghost hex
 #pragma unroll
       for (int k = 0; k < 3; ++k) w2[k] = v2d{0.0, 0.0};
       continue;
     }
     double zc[8];  // corners of the hex whose lower corner is lattice point (I-1, J-1, K-1)
 #pragma unroll
-    for (int c = 0; c < 8; ++c) zc[c] = A.zlat[hp + (c & 1) + px * (((c >> 1) & 1) + (size_t)py * (c >> 2))];
+    for (int c = 0; c < 8; ++c) zc[c] = A.zlat[hp + (c & 1) + px * (((c >> 1) & 1) + py * (c >> 2))];
     double w6[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
-      const uint8_t* cr = A.corners + 4 * t;
-      const double rho = (zc[cr[0]] + zc[cr[1]] + zc[cr[2]] + zc[cr[3]]) / 4.0;
+      const double rho = (zc[cr[4 * t]] + zc[cr[4 * t + 1]] + zc[cr[4 * t + 2]] + zc[cr[4 * t + 3]]) / 4.0;
       // SIMP with a small integer exponent: penalty_w's own multiplication path, inlined
       // without the other models' code
       w6[t] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);

@@ -1208,12 +1208,13 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
     h->box_lam = a.m.c01;
     h->box_mu = a.m.c33;
   }
-  const size_t nh = (size_t)(a.nx + 2) * (a.ny + 2) * (a.nz + 2);
+  // k_box_w: one wave per run of 64 padded hexes along x
+  const size_t wv = (size_t)((a.nx + 2 + 63) / 64) * (a.ny + 2) * (a.nz + 2);
   pa::k_box_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, h->stream>>>(a);
   if (a.m.model == pa::kPenaltySimp && a.m.pint > 0)
-    pa::k_box_w<true><<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
+    pa::k_box_w<true><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a);
   else
-    pa::k_box_w<false><<<grid_for(nh, 256, 16384), 256, 0, h->stream>>>(a);
+    pa::k_box_w<false><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a);
   const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
   const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, R ? h->box_grid_r : h->box_grid_k);
   if (R)
