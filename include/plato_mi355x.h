@@ -186,8 +186,9 @@ int pa_solve(pa_handle h, const double* values, double* x, const double* b,
  * precision 1: Jacobi-PCG with FP64 vectors and reductions whose operator K p is applied
  * matrix-free in FP32; each time the iterated residual has dropped by inner_tolerance it
  * is replaced by the true FP64 residual f - K u of the FP64 element kernel (the search
- * direction is kept); convergence is judged on that true residual.  use_x0 = 1 starts from the caller's u
- * (warm start; its constrained dofs are reset to their prescribed values). */
+ * direction is kept); convergence is judged on that true residual.  use_x0 = 1 (either
+ * precision, either preconditioner) starts from the caller's u (warm start; its
+ * constrained dofs are reset to their prescribed values). */
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
 

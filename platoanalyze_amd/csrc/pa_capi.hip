@@ -1762,7 +1762,11 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   if (int rc = sub_load(h, rhs, 1.0)) return rc;
   if (int rc = assemble_jacobian(h, nullptr, z, vals, nullptr)) return rc;
   if (int rc = pa_apply_constraints(h, vals, rhs, 1.0)) return rc;
-  opts.use_x0 = 0;
+  if (opts.use_x0) {  // warm start: the caller's u, its constrained dofs at their prescribed values
+    const int nbc = (int)h->bc_dofs.size();
+    if (nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, h->stream>>>(u, h->bc_d.p, h->bc_v.p, nbc, 1.0);
+    PA_HIP(hipGetLastError());
+  }
   if (mg) return solve_mg(h, z, u, rhs, vals, opts, st, false);
   return pa_solve(h, vals, u, rhs, &opts, st);
 }

@@ -285,7 +285,8 @@ class Problem:
         self.state = None
 
     def solution(self, z: torch.Tensor, u0: torch.Tensor | None = None) -> torch.Tensor:
-        """Problem::solution; `u0` warm-starts the solve (mixed precision only)."""
+        """Problem::solution; `u0` warm-starts the solve (any precision and preconditioner;
+        its constrained dofs are reset to their prescribed values)."""
         if u0 is not None:
             u = u0.detach().clone().contiguous()
             self.opts.use_x0 = 1

@@ -5,10 +5,13 @@
 
 Data: one multigrid solve per OC iteration (random volume fraction and traction
 direction per trajectory); the last trajectory is held out.  Evaluation: on the held-out
-designs, the mixed-precision solve to 1e-8 from zero against the same solve from the
-network's warm start (iterations and seconds), both judged by the true FP64 residual;
+designs, the mixed-precision (or, with --eval-precision fp64, the FP64) block-Jacobi PCG
+solve to 1e-8 from zero against the same solve from the network's warm start (iterations
+and seconds), both judged by the true FP64 residual;
 in update mode also from the previous design's solution, with Jacobi and multigrid
-preconditioning.
+preconditioning.  `--apply WORKLOAD` re-instantiates the trained network on that
+workload's node grid (the network is fully convolutional) and compares mixed multigrid
+solves of its design from zero and from the network's guess.
 """
 from __future__ import annotations
 
@@ -21,7 +24,7 @@ import numpy as np
 import torch
 
 
-def evaluate(net, c0, held, tol=1e-8, log=print, mode=None):
+def evaluate(net, c0, held, tol=1e-8, log=print, mode=None, precision="mixed"):
     from . import workloads as wl
     from .elliptic import Problem
     from .warmstart import WarmStart
@@ -30,7 +33,7 @@ def evaluate(net, c0, held, tol=1e-8, log=print, mode=None):
     for w, z, f in held:
         m = wl.model(w)
         ws = WarmStart(net, w, m, c0, mode=mode)
-        pb = Problem(m, tolerance=tol, check_every=16, precision="mixed")
+        pb = Problem(m, tolerance=tol, check_every=16, precision=precision)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         u_cold = pb.solution(z)
@@ -76,6 +79,41 @@ def evaluate_update(net, c0, held, tol=1e-8, log=print, preconditioner="jacobi")
     return rows
 
 
+def evaluate_transfer(net, c0, mode, config, tol=1e-8, log=print):
+    """The trained network (fully convolutional) re-instantiated with the same parameters
+    on another box's node grid and applied to that workload's design: mixed multigrid
+    solves from zero and from the network's guess (iterations, seconds including the
+    network's forward pass, the guess's error against the solution)."""
+    from . import workloads as wl
+    from .elliptic import Problem
+    from .unet import UNet
+    from .warmstart import Features, WarmStart
+
+    w = wl.make(config)
+    m = wl.model(w)
+    big = UNet(tuple(Features(w).dims), net.base)
+    big.set_params(net.params())
+    ws = WarmStart(big, w, m, c0, mode=mode)
+    z = torch.as_tensor(w.z, dtype=torch.float64, device="cuda")
+    pb = Problem(m, tolerance=tol, check_every=4, precision="mixed", preconditioner="multigrid")
+    pb.solution(z)  # builds the multigrid hierarchy (once per mesh)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u = pb.solution(z).clone()
+    torch.cuda.synchronize()
+    t_cold, it_cold = time.perf_counter() - t0, pb.stats.iterations
+    t0 = time.perf_counter()
+    u0 = ws.initial_guess(z)
+    u_w = pb.solution(z, u0=u0)
+    torch.cuda.synchronize()
+    t_warm, it_warm = time.perf_counter() - t0, pb.stats.iterations
+    row = {"config": w.cfg.name, "grid": list(big.grid), "mode": ws.mode, "iters_cold": it_cold,
+           "iters_warm": it_warm, "s_cold": t_cold, "s_warm": t_warm,
+           "u0_rel_err": float((u0 - u).norm() / u.norm()), "u_rel_diff": float((u_w - u).norm() / u.norm())}
+    log(json.dumps(row))
+    return row
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cantilever_64x32x32")
@@ -89,6 +127,10 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mode", default="load", choices=["load", "shape", "basis", "update"])
     ap.add_argument("--out", required=True)
+    ap.add_argument("--eval-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="solver of the held-out evaluation (fp64: deterministic iteration counts)")
+    ap.add_argument("--apply", default=None,
+                    help="also apply the trained network, fully convolutionally, to this workload")
     a = ap.parse_args(argv)
 
     from . import workloads as wl
@@ -108,10 +150,15 @@ def main(argv=None) -> int:
             log(f"evaluation, {pre} preconditioner")
             rows += evaluate_update(net, c0, [held[i] for i in idx], log=log, preconditioner=pre)
     else:
-        rows = evaluate(net, c0, [held[i] for i in idx], log=log, mode=a.mode)
+        rows = evaluate(net, c0, [held[i] for i in idx], log=log, mode=a.mode, precision=a.eval_precision)
     meta = {"config": base.cfg.name, "mode": a.mode, "traj": a.traj, "iters": a.iters, "epochs": a.epochs,
             "loss_history": np.array(hist), "eval_iters_cold": np.array([r["iters_cold"] for r in rows]),
             "eval_iters_warm": np.array([r["iters_warm"] for r in rows])}
+    if a.apply:
+        if a.mode == "update":
+            raise SystemExit("--apply needs a mode that predicts from the design (load, shape or basis)")
+        log(f"applied to {a.apply}")
+        evaluate_transfer(net, c0, a.mode, a.apply, log=log)
     save(net, c0, a.out, meta)
     log(json.dumps({"saved": a.out, "final_loss": hist[-1] if hist else None,
                     "mean_iters_cold": float(np.mean([r["iters_cold"] for r in rows])),

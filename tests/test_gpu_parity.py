@@ -314,6 +314,41 @@ def test_gpu_mixed_precision_solve_cfg0():
     assert mp2.stats.iterations == 0 and mp2.stats.converged
 
 
+@pytest.mark.parametrize("preconditioner", ["jacobi", "multigrid"])
+def test_gpu_fp64_warm_start(preconditioner):
+    """FP64 solves honour an initial guess: from the converged state no iteration is
+    needed; from a perturbed state (constrained dofs included) the solve reaches the same
+    solution with its prescribed values restored."""
+    print(f"case: fp64 warm start {preconditioner}", flush=True)
+    mesh = build_box(8, 4, 4, 2.0, 1.0, 1.0)
+    fixed = np.flatnonzero(mesh.coords[:, 0] == 0.0).astype(np.int32)
+    bcs = [EssentialBC(fixed, 0, 0.0), EssentialBC(fixed, 1, 0.01), EssentialBC(fixed, 2, -0.02)]
+    from platoanalyze_amd.mesh import faces_on_plane
+    fv, fe = faces_on_plane(mesh, 0, 2.0)
+    m = ElasticModel(mesh, 1.0, 0.3, 3.0, 1e-3, bcs, [Traction(fv, fe, (0.0, 0.0, -1.0))], box=(8, 4, 4))
+    z = dev(np.random.default_rng(5).uniform(0.2, 1.0, mesh.num_nodes))
+    pb = Problem(m, tolerance=1e-11, preconditioner=preconditioner)
+    u = pb.solution(z).clone()
+    cold = pb.stats.iterations
+    assert pb.stats.converged and cold > 0
+    # a looser tolerance than the one u was solved to: its true residual is below it
+    loose = Problem(m, tolerance=1e-9, preconditioner=preconditioner)
+    loose.solution(z, u0=u)
+    assert loose.stats.converged and loose.stats.iterations == 0
+    rng = np.random.default_rng(6)
+    u0 = u + dev(1e-3 * rng.normal(size=m.num_dofs))
+    uw = host(pb.solution(z, u0=u0))
+    assert pb.stats.converged and pb.stats.iterations > 0
+    assert np.abs(uw[3 * fixed + 1] - 0.01).max() <= 1e-10
+    assert np.abs(uw[3 * fixed + 2] + 0.02).max() <= 1e-10
+    uc = host(u)
+    # both solutions have relative residuals below 1e-11; on this 675-dof problem they
+    # agree far inside 1e-6
+    err = np.abs(uw - uc).max() / np.abs(uc).max()
+    print(f"cold {cold} it, perturbed warm start {pb.stats.iterations} it, difference {err:.2e}", flush=True)
+    assert err <= 1e-6
+
+
 def test_gpu_mixed_precision_nonzero_dirichlet():
     mesh = build_box(8, 4, 4, 2.0, 1.0, 1.0)
     fixed = np.flatnonzero(mesh.coords[:, 0] == 0.0).astype(np.int32)
