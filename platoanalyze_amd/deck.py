@@ -43,7 +43,10 @@ import torch
 
 from .elliptic import ElasticModel, EssentialBC, PhysicsScalarFunction, Problem, Traction, Volume, VectorFunction
 from .elliptic import apply_constraints, solve
+from ._capi import PaError
 from .exodus import ExodusMesh
+
+PA_ERR_MESH = 3  # include/plato_mi355x.h
 
 _MISSING = object()
 
@@ -191,7 +194,9 @@ class ElasticProblem:
         pre = str(ls.get("Preconditioner", "Jacobi")).lower()
         if precision not in ("fp64", "mixed") or pre not in ("jacobi", "multigrid"):
             raise ValueError(f"Linear Solver: Precision {precision!r} / Preconditioner {pre!r} not supported")
-        self.box = infer_box(ex.mesh) if pre == "multigrid" else None
+        # a box mesh is declared whatever the solver: it also selects the lattice assembly
+        self._pre = pre
+        self.box = infer_box(ex.mesh)
         if pre == "multigrid" and self.box is None:
             raise ValueError("Linear Solver: Multigrid needs a structured box mesh (tensor lattice, 6 tets per hex)")
         self.model = self._handle(self.penalty, with_box=True)
@@ -279,8 +284,21 @@ class ElasticProblem:
 
     def _handle(self, penalty, with_box: bool = False) -> ElasticModel:
         typ, p, rmin, length = penalty
-        return ElasticModel(self.mesh.mesh, self.E, self.nu, p, rmin, self.bcs, self.tractions, device=self.device,
-                            box=self.box if with_box else None, penalty_model=typ, regularization_length=length)
+
+        def make(box):
+            return ElasticModel(self.mesh.mesh, self.E, self.nu, p, rmin, self.bcs, self.tractions,
+                                device=self.device, box=box, penalty_model=typ, regularization_length=length)
+
+        if not (with_box and self.box):
+            return make(None)
+        try:
+            return make(self.box)
+        except PaError as e:
+            # a tensor lattice with unequal spacings is no box: only the multigrid solve needs one
+            if self._pre == "multigrid" or e.status != PA_ERR_MESH:
+                raise
+            self.box = None
+            return make(None)
 
     def _criteria(self, pp: ParameterList) -> dict:
         out = {}

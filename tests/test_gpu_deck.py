@@ -228,6 +228,38 @@ def test_gpu_deck_matches_direct_api():
         pb.criterionValue(dz, u, "nope")
 
 
+def _box_active(m) -> int:
+    import ctypes as C
+    v = C.c_int(-1)
+    _capi.call("pa_box_assembly", m.handle, C.byref(v))
+    return v.value
+
+
+def test_gpu_deck_declares_box_for_every_solver():
+    """A box mesh is declared whatever the deck's solver, so the lattice assembly applies
+    under block Jacobi too; a tensor lattice with unequal spacings is no box: a Jacobi deck
+    still builds (general assembly) and matches the direct API, a multigrid deck is refused."""
+    print("case: deck box declaration", flush=True)
+    from platoanalyze_amd.elliptic import ElasticModel, EssentialBC, Traction
+    from platoanalyze_amd.mesh import TetMesh
+    w, ex, z, _ = _cantilever_mesh()
+    dz = dev(z)
+    pb = ElasticProblem(parse_parameter_list(_cantilever_deck(True)), ex)
+    assert pb.box == (4, 2, 2) and _box_active(pb.model) == 1
+    X = w.mesh.coords.copy()
+    X[:, 0] = X[:, 0] ** 1.3 / 2.0 ** 0.3  # graded along x, still spanning [0, 2]
+    graded = TetMesh(X, w.mesh.conn)
+    ex2 = ExodusMesh(graded, ex.node_sets, ex.side_sets, {"body": (0, graded.num_elems)})
+    pb2 = ElasticProblem(parse_parameter_list(_cantilever_deck(True)), ex2)
+    assert pb2.box is None and _box_active(pb2.model) == 0
+    bcs = [EssentialBC(w.fixed_nodes, d, 0.0) for d in range(3)]
+    ref = ElasticModel(graded, 1.0, 0.3, 3.0, 1e-3, bcs, [Traction(w.face_verts, w.face_elems, (0.0, 0.0, -1.0))])
+    assert rel(host(pb2.solution(dz)), host(Problem(ref, tolerance=1e-12).solution(dz))) <= 1e-10
+    mg = _cantilever_deck(True, solver="<Parameter name='Preconditioner' type='string' value='Multigrid'/>")
+    with pytest.raises(_capi.PaError):
+        ElasticProblem(parse_parameter_list(mg), ex2)
+
+
 def _oracle_compliance(coords, w, z, rm, ci):
     o = orc.Model(coords, w.mesh.conn, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min, w.face_verts, w.traction)
     f = -o.residual(np.zeros(3 * w.mesh.num_nodes), z)
