@@ -45,6 +45,7 @@ from .elliptic import ElasticModel, EssentialBC, PhysicsScalarFunction, Problem,
 from .elliptic import apply_constraints, solve
 from ._capi import PaError
 from .exodus import ExodusMesh
+from .mesh import infer_box  # noqa: F401  (part of this module's interface)
 
 PA_ERR_MESH = 3  # include/plato_mi355x.h
 
@@ -148,17 +149,6 @@ def penalty_of(params: ParameterList | None, default_type: str = "SIMP") -> tupl
     p0, m0 = _PENALTY_DEFAULTS[typ]
     return (typ, float(params.get("Exponent", p0)), float(params.get("Minimum Value", m0)),
             float(params.get("Regularization Length", 1.0)))
-
-
-def infer_box(mesh) -> tuple[int, int, int] | None:
-    """(nx, ny, nz) when the nodes form a full tensor lattice with 6 tets per hex."""
-    X = mesh.coords
-    n = [len(np.unique(X[:, d])) - 1 for d in range(3)]
-    if min(n) < 1:
-        return None
-    if (n[0] + 1) * (n[1] + 1) * (n[2] + 1) != mesh.num_nodes or 6 * n[0] * n[1] * n[2] != mesh.num_elems:
-        return None
-    return (n[0], n[1], n[2])
 
 
 class _Criterion:

@@ -208,8 +208,12 @@ def model_from_exodus(ex: ExodusMesh, E: float = 1.0, nu: float = 0.3, penalty_e
                       penalty_minimum: float = 1e-9, essential: dict | None = None, tractions: dict | None = None,
                       **kw):
     """ElasticModel from named sets: essential = {node set: [(dof, value), ...]},
-    tractions = {side set: (tx, ty, tz)}."""
+    tractions = {side set: (tx, ty, tz)}.  A mesh whose nodes form a uniform box lattice
+    is declared a box (lattice assembly, multigrid available) unless `box` is given;
+    box=None declares none."""
+    from ._capi import PaError
     from .elliptic import ElasticModel, EssentialBC, Traction
+    from .mesh import infer_box
 
     bcs = []
     for name, dofs in (essential or {}).items():
@@ -223,4 +227,14 @@ def model_from_exodus(ex: ExodusMesh, E: float = 1.0, nu: float = 0.3, penalty_e
             raise KeyError(f"side set {name!r} not in the mesh ({sorted(ex.side_sets)})")
         fv, fe = ex.side_sets[name]
         trs.append(Traction(fv, fe, tuple(float(v) for v in t)))
+    box = kw.pop("box", "auto")
+    if box != "auto":
+        return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, box=box, **kw)
+    box = infer_box(ex.mesh)
+    if box is not None:
+        try:
+            return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, box=box, **kw)
+        except PaError as e:
+            if e.status != 3:  # PA_ERR_MESH: a tensor lattice with unequal spacings is no box
+                raise
     return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, **kw)

@@ -91,3 +91,16 @@ def node_graph(mesh: TetMesh):
     ci = np.empty(nnz.value, np.int32)
     _capi.call("pa_node_graph", mesh.num_nodes, mesh.num_elems, _ptr(conn), _ptr(rm), _ptr(ci), C.byref(nnz))
     return rm, ci
+
+
+def infer_box(mesh) -> tuple[int, int, int] | None:
+    """(nx, ny, nz) when the nodes form a full tensor lattice with 6 tets per hex (the
+    library still checks, when the box is declared, that the lattice is uniform and every
+    hex carries the same six tets)."""
+    X = mesh.coords
+    n = [len(np.unique(X[:, d])) - 1 for d in range(3)]
+    if min(n) < 1:
+        return None
+    if (n[0] + 1) * (n[1] + 1) * (n[2] + 1) != mesh.num_nodes or 6 * n[0] * n[1] * n[2] != mesh.num_elems:
+        return None
+    return (n[0], n[1], n[2])

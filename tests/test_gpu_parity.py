@@ -551,3 +551,20 @@ def test_gpu_model_from_exodus(tmp_path):
     u = dev(np.random.default_rng(2).normal(size=3 * w.mesh.num_nodes))
     close(host(VectorFunction(m).value(u, z)), host(VectorFunction(ref).value(u, z)), RTOL, "residual")
     close(host(Problem(m, tolerance=1e-12).solution(z)), host(Problem(ref, tolerance=1e-12).solution(z)), 1e-10, "u")
+    # the file's box lattice is declared: lattice assembly and multigrid apply
+    import ctypes as C
+
+    from platoanalyze_amd import _capi
+    v = C.c_int(-1)
+    _capi.call("pa_box_assembly", m.handle, C.byref(v))
+    assert v.value == 1
+    close(host(Problem(m, tolerance=1e-10, preconditioner="multigrid").solution(z)),
+          host(Problem(ref, tolerance=1e-12).solution(z)), 1e-7, "u (multigrid)")
+    # a graded lattice is no box: the general assembly, the same residual as a direct model
+    X = w.mesh.coords.copy()
+    X[:, 0] = X[:, 0] ** 1.3 / 2.0 ** 0.3
+    ex.mesh = TetMesh(X, w.mesh.conn)
+    mg = model_from_exodus(ex, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min,
+                           essential={"x-": [(0, 0.0), (1, 0.0), (2, 0.0)]}, tractions={"x+": w.traction})
+    _capi.call("pa_box_assembly", mg.handle, C.byref(v))
+    assert v.value == 0
