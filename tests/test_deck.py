@@ -88,6 +88,11 @@ def test_infer_box():
     moved = build_box(2, 2, 2, 1.0, 1.0, 1.0)
     moved.coords[4, 0] += 0.01
     assert infer_box(moved) is None
+    # a graded tensor lattice is still a candidate: the library's box check (uniform
+    # spacing) decides, and the callers fall back to the general path when it refuses
+    graded = build_box(4, 3, 2, 2.0, 1.0, 1.0)
+    graded.coords[:, 0] = graded.coords[:, 0] ** 1.3 / 2.0 ** 0.3
+    assert infer_box(graded) == (4, 3, 2)
 
 
 APP = """
