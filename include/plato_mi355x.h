@@ -235,6 +235,12 @@ int pa_compliance_gradient(pa_handle h, const double* u, const double* z, double
 int pa_stress_pnorm(pa_handle h, const double* u, const double* z, int measure, double p, double q,
                     double* value, double* dfdu, double* dfdz);
 
+/* PhysicsScalarFunction::gradient_x of the stress p-norm (src/elliptic/PhysicsScalarFunction.hpp:293-355
+ * with StressPNorm, src/elliptic/StressPNorm.hpp:88-142): partial dF/dX at fixed u and z
+ * (3N), including the cell volumes' dependence on the coordinates; value (host) may be NULL. */
+int pa_stress_pnorm_gradient_x(pa_handle h, const double* u, const double* z, int measure, double p, double q,
+                               double* dfdx, double* value);
+
 /* (dR/dz)^T lambda per control node (VectorFunction::gradient_z applied transposed,
  * the MatrixTimesVectorPlusVector term of src/elliptic/Problem.hpp:437-466) without
  * forming dR/dz. */

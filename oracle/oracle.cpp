@@ -22,15 +22,21 @@
 //   orc_graph ........................ CreateBlockMatrix rule; row lengths = vertex degree + 1
 //   orc_apply_constraints / orc_pcg .. dense LAPACK solve on small meshes (numpy)
 //   orc_traction ..................... total force = area * traction (closed form)
-//   orc_stress_pnorm ................. DerivativeTests.cpp:1058 (14525.25169157 at p=12)
+//   orc_stress_pnorm / orc_pnorm ..... DerivativeTests.cpp:1058 (14525.25169157 at p=12),
+//                                      :1069-1108 (grad_u 27, grad_z 27)
+//   orc_pnorm_gradient_x ............. DerivativeTests.cpp:1113-1137 (27 values at p=12)
+//   vm_of / orc_von_mises ............ PlatoAugLagStressTest.cpp:1063-1101 (two cells)
 //   orc_gradient_x ................... DerivativeTests.cpp:469-483 (36 entries); central
 //                                      differences of orc_residual in the coordinates
 //   orc_energy_gradient_x ............ DerivativeTests.cpp:665-693 (81 values); central
 //                                      differences of orc_energy in the coordinates
-//   penalty models (RAMP, Heaviside, none) and orc_volume ... parity unpinned by the
-//                                      reference (no elliptic goldens): closed forms
-//                                      (box volume, limits of each model) and central
-//                                      differences in the densities and coordinates
+//   orc_volume ....................... DerivativeTests.cpp:2151-2302 (value 1, dV/dz 27,
+//                                      dV/dX 81) with linear SIMP; closed forms per model
+//   penalty weights at interior densities (SIMP p=3 rmin=1e-3, RAMP, Heaviside, none):
+//                                      hand-computed w(c) of src/Simp.hpp:130-134,
+//                                      src/Ramp.hpp:94-100, src/Heaviside.hpp:110-125
+//                                      against K(z=c) = w(c) K(1) on the whole mesh and
+//                                      K(z) = w(mean z) K(1) on one element
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -803,7 +809,9 @@ R pnorm_cell(const Params& P, const S* eps, const Z* zc, double vol, int measure
   for (int n = 0; n < 4; ++n) rho += zc[n];
   rho = rho / 4.0;
   if (measure == 0) {
-    Z w = P.rmin + (1.0 - P.rmin) * fad_pow(rho, P.p);
+    // the criterion's own "Penalty Function" (src/Mechanics.hpp:241-276 instantiates
+    // StressPNorm per penalty model)
+    Z w = P.model != 0 ? penalty_of(P, rho) : P.rmin + (1.0 - P.rmin) * fad_pow(rho, P.p);
     R ss(0.0);
     for (int k = 0; k < 6; ++k) {
       R t = mix<R>(s[k], w);
@@ -886,6 +894,58 @@ double orc_pnorm(int nn, int ne, const double* X, const int* conn, const double*
 double orc_stress_pnorm(int nn, int ne, const double* X, const int* conn, const double* u, const double* z,
                         double E, double nu, double p, double rmin, double pnorm) {
   return orc_pnorm(nn, ne, X, conn, u, z, E, nu, p, rmin, 0, pnorm, 0.0, nullptr, nullptr);
+}
+
+// VonMisesYieldFunction<3> (src/VonMisesYieldFunction.hpp:53-74) of n Voigt stresses.
+void orc_von_mises(int n, const double* stress6, double* out) {
+  for (int c = 0; c < n; ++c) out[c] = vm_of(stress6 + 6 * (size_t)c);
+}
+
+// PhysicsScalarFunction::gradient_x of the stress p-norm criteria
+// (src/elliptic/PhysicsScalarFunction.hpp:293-355): the cell coordinates seeded as
+// SFad<12> through ComputeGradientWorkset, strain, stress, weighting and the cell
+// volume (src/elliptic/StressPNorm.hpp:88-142), then the TensorNormBase::postEvaluate
+// scaling pow(S,(1-p)/p)/p of the assembled gradient (src/TensorPNorm.hpp:295-305).
+double orc_pnorm_gradient_x(int nn, int ne, const double* X, const int* conn, const double* u, const double* z,
+                            double E, double nu, double p, double rmin, int measure, double pn, double q,
+                            double* gx) {
+  using F = Fad<12>;
+  Mesh m{nn, ne, X, conn};
+  Params P;
+  make_params(E, nu, p, rmin, P);
+  auto cfgw = workset_config(m);
+  std::memset(gx, 0, sizeof(double) * 3 * (size_t)nn);
+  double S = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : S)
+  for (int e = 0; e < ne; ++e) {
+    F cfg[12], g[12], vol;
+    double ue[12], zc[4];
+    for (int k = 0; k < 12; ++k) cfg[k] = F(cfgw[12 * (size_t)e + k], k);
+    for (int n = 0; n < 4; ++n) {
+      zc[n] = z[conn[4 * e + n]];
+      for (int d = 0; d < 3; ++d) ue[3 * n + d] = u[3 * (size_t)conn[4 * e + n] + d];
+    }
+    gradient_workset_t(cfg, g, vol);
+    double rho = 0.0;
+    for (int n = 0; n < 4; ++n) rho += zc[n];
+    rho = rho / 4.0;
+    F eps[6], sw[6], val;
+    if (measure == 0) {
+      element_stress_t(P, g, ue, msimp_weight(P, zc), eps, sw);
+      F ss(0.0);
+      for (int k = 0; k < 6; ++k) ss += sw[k] * sw[k];
+      val = fad_pow(ss, (int)pn / 2.0) * vol;
+    } else {
+      element_stress_t(P, g, ue, std::pow(rho, q), eps, sw);
+      val = fad_pow(vm_of(sw), pn) * vol;
+    }
+    S += val.v;
+    for (int k = 0; k < 12; ++k) atomic_add(&gx[3 * (size_t)conn[4 * e + k / 3] + k % 3], val.dx[k]);
+  }
+  const double pe = measure == 0 ? (double)(int)pn : pn;
+  const double scale = std::pow(S, (1.0 - pe) / pe) / pe;
+  for (size_t k = 0; k < 3 * (size_t)nn; ++k) gx[k] *= scale;
+  return std::pow(S, 1.0 / pe);
 }
 
 // (dR/dz)^T lambda per control node: sum_e dw/drho/4 * lambda_e . (vol B^T sigma0(u_e))

@@ -23,6 +23,7 @@ def lib():
         _lib.orc_stress_pnorm.restype = C.c_double
         _lib.orc_pnorm.restype = C.c_double
         _lib.orc_volume.restype = C.c_double
+        _lib.orc_pnorm_gradient_x.restype = C.c_double
     return _lib
 
 
@@ -144,6 +145,13 @@ class Model:
                             C.c_double(q), _p(gu), _p(gz))
         return float(v), gu, gz
 
+    def pnorm_gradient_x(self, u, z, measure, pn, q=0.0):
+        """dF/dX of the stress p-norm at fixed u and z (3N); returns (F, dF/dX)."""
+        g = np.empty(3 * self.nn)
+        v = lib().orc_pnorm_gradient_x(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(), int(measure),
+                                       C.c_double(pn), C.c_double(q), _p(g))
+        return float(v), g
+
     def dRdz_T_apply(self, u, lam, z):
         out = np.empty(self.nn)
         lib().orc_dRdz_T_apply(*self._m(), _p(_f64(u)), _p(_f64(lam)), _p(_f64(z)), *self._mat(), _p(out))
@@ -152,6 +160,14 @@ class Model:
     def stress_pnorm(self, u, z, pnorm):
         return float(lib().orc_stress_pnorm(*self._m(), _p(_f64(u)), _p(_f64(z)), *self._mat(),
                                             C.c_double(pnorm)))
+
+
+def von_mises(stress6):
+    """VonMisesYieldFunction<3> of Voigt stresses [n][6]."""
+    s = _f64(stress6).reshape(-1, 6)
+    out = np.empty(s.shape[0])
+    lib().orc_von_mises(s.shape[0], _p(s), _p(out))
+    return out
 
 
 def apply_constraints(rowmap, colidx, values, rhs, dofs, bcvals, scale=1.0):

@@ -18,6 +18,9 @@ STATUS_NAMES = {0: "PA_OK", 1: "PA_ERR_ARG", 2: "PA_ERR_HIP", 3: "PA_ERR_MESH", 
                 5: "PA_ERR_NOT_CONVERGED", 6: "PA_ERR_UNSUPPORTED"}
 
 
+PA_ERR_MESH = 3  # include/plato_mi355x.h: mesh/graph inconsistent
+
+
 class PaError(RuntimeError):
     def __init__(self, status: int, where: str, msg: str):
         super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}: {msg}")
@@ -72,6 +75,7 @@ _SIGS = {
     "pa_solution": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "pa_compliance_gradient": (_I, [_P, _P, _P, _P, _PD]),
     "pa_stress_pnorm": (_I, [_P, _P, _P, _I, _D, _D, _PD, _P, _P]),
+    "pa_stress_pnorm_gradient_x": (_I, [_P, _P, _P, _I, _D, _D, _P, _PD]),
     "pa_dRdz_T_apply": (_I, [_P, _P, _P, _P, _P]),
     "pa_matvec_plus_vector": (_I, [_P, C.c_int, C.c_double, _P, _P, _P]),
     "pa_stress_pnorm_gradient": (_I, [_P, _P, _P, _I, _D, _D, _P, _P, _PD, C.POINTER(SolverOpts),

@@ -15,7 +15,7 @@ import ctypes as C
 import torch
 
 from . import _capi
-from .elliptic import Problem, _dptr
+from .elliptic import Problem, _dptr, _dvec
 
 MEASURES = {"voigt": 0, "von_mises": 1}
 
@@ -31,23 +31,36 @@ class StressPNorm:
 
     def value(self, u, z) -> float:
         v = C.c_double()
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_stress_pnorm", self.m.handle, _dptr(u), _dptr(z), self.measure, self.p, self.q,
+        _capi.call("pa_stress_pnorm", self.m.handle, up, zp, self.measure, self.p, self.q,
                    C.byref(v), None, None)
         return v.value
 
     def gradient_u(self, u, z):
         g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_stress_pnorm", self.m.handle, _dptr(u), _dptr(z), self.measure, self.p, self.q, None,
+        _capi.call("pa_stress_pnorm", self.m.handle, up, zp, self.measure, self.p, self.q, None,
                    _dptr(g), None)
         return g
 
     def gradient_z(self, u, z):
         g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_stress_pnorm", self.m.handle, _dptr(u), _dptr(z), self.measure, self.p, self.q, None,
+        _capi.call("pa_stress_pnorm", self.m.handle, up, zp, self.measure, self.p, self.q, None,
                    None, _dptr(g))
+        return g
+
+    def gradient_x(self, u, z):
+        """Partial dF/dX at fixed u and z (3N): PhysicsScalarFunction::gradient_x
+        (src/elliptic/PhysicsScalarFunction.hpp:293-355) of the p-norm."""
+        g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
+        self.m._stream()
+        _capi.call("pa_stress_pnorm_gradient_x", self.m.handle, up, zp, self.measure, self.p, self.q,
+                   _dptr(g), None)
         return g
 
     def total_gradient(self, u, z, problem: Problem):
@@ -55,16 +68,19 @@ class StressPNorm:
         g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=u.device)
         lam = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
         v = C.c_double()
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_stress_pnorm_gradient", self.m.handle, _dptr(u), _dptr(z), self.measure, self.p, self.q,
+        _capi.call("pa_stress_pnorm_gradient", self.m.handle, up, zp, self.measure, self.p, self.q,
                    _dptr(g), _dptr(lam), C.byref(v), C.byref(problem.opts), C.byref(problem.stats))
         return g, v.value, lam
 
 
 def dRdz_T_apply(model, u, lam, z):
+    up, zp = model._uz(u, z)
+    lp = _dvec(lam, model.num_dofs, "lambda")
     out = torch.empty(model.num_nodes, dtype=torch.float64, device=u.device)
     model._stream()
-    _capi.call("pa_dRdz_T_apply", model.handle, _dptr(u), _dptr(lam), _dptr(z), _dptr(out))
+    _capi.call("pa_dRdz_T_apply", model.handle, up, lp, zp, _dptr(out))
     return out
 
 

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -732,6 +733,7 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   double rel = bb > 0 ? std::sqrt(h->host_scal[pa::kRR] / bb) : 0.0;
   int it = 0;
   bool conv = bb == 0.0 || rel <= opts.tolerance;
+  if (bb == 0.0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);  // b = 0: the solution is 0, whatever x0
   while (!conv && it < opts.max_iters) {
     // rz of this iteration in P_RZ[cur]; the update writes the next one into P_RZ[cur ^ 1]
     const int cur = it & 1;
@@ -871,6 +873,7 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   rel = bb > 0 ? std::sqrt(rr_ref / bb) : 0.0;
   if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "mixed-precision solve: non-finite residual");
   conv = bb == 0.0 || rel <= opts.tolerance;
+  if (bb == 0.0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(u, ndof, 0.0);  // f = 0: the solution is 0, whatever u0
   if (!conv) {
     PA_HIP(pa::launch_pcgm_init(h->m_r.p, h->m_z.p, h->m_dinv.p, nn, PS(pa::kPRZ0), G, s));
     PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), nullptr, G, h->scal.p, pa::kBetaInit, s));
@@ -934,12 +937,20 @@ static bool box_grid(const double* X, int nn, const pa::BoxDims& d, const double
   grid.assign(3 * (size_t)nn, 0);
   std::vector<uint8_t> seen((size_t)d.nx * d.ny * d.nz, 0);
   const int n3[3] = {d.nx, d.ny, d.nz};
+  // Tolerance in spacings: 1e-6, or the coordinates' own round-off where that is larger
+  // (lattices stored in single precision round by ~6e-8 of the largest coordinate).
+  double tol[3];
+  for (int a = 0; a < 3; ++a) {
+    const double xmax = std::max(std::fabs(lo[a]), std::fabs(lo[a] + len[a]));
+    tol[a] = std::max(1e-6, 16.0 * 6e-8 * xmax * (n3[a] - 1) / len[a]);
+    if (tol[a] >= 0.25) return false;  // coordinates too coarse to resolve the lattice
+  }
   for (int n = 0; n < nn; ++n) {
     for (int a = 0; a < 3; ++a) {
       const double hstep = len[a] / (n3[a] - 1);
       const double t = (X[3 * (size_t)n + a] - lo[a]) / hstep;
       const double r = std::nearbyint(t);
-      if (std::fabs(t - r) > 1e-6 || r < 0 || r > n3[a] - 1) return false;
+      if (!(std::fabs(t - r) <= tol[a]) || r < 0 || r > n3[a] - 1) return false;  // NaN fails too
       grid[3 * (size_t)n + a] = (int32_t)r;
     }
     const size_t g = ((size_t)grid[3 * (size_t)n] * d.ny + grid[3 * (size_t)n + 1]) * d.nz + grid[3 * (size_t)n + 2];
@@ -962,6 +973,9 @@ int pa_set_box(pa_handle h, int nx, int ny, int nz) {
     }
   const pa::BoxDims d{nx + 1, ny + 1, nz + 1};
   double len[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  for (int a = 0; a < 3; ++a)
+    if (!(len[a] > 0.0) || !std::isfinite(len[a]) || !std::isfinite(lo[a]))
+      return fail(PA_ERR_MESH, "pa_set_box: the box has a zero-length or non-finite axis");
   std::vector<int32_t> grid;
   if (!box_grid(h->coords.data(), h->nn, d, lo, len, grid))
     return fail(PA_ERR_MESH, "pa_set_box: the nodes do not form the box's node lattice");
@@ -999,7 +1013,8 @@ static void host_tet_gradients(const double x[4][3], double g[4][3], double& vol
 }
 
 // Decide whether the lattice assembly applies and build its tables.  Conditions: at most
-// 1024 nodes per axis; every node within 1e-13 spacings of its lattice point; every
+// 1024 nodes per axis; every node within max(1e-13, 8 eps |x|max / step) spacings of its
+// lattice point; every
 // element inside one hex, the elements of every hex being the same six corner sets; every
 // graph row at most 16 blocks, all within one lattice step of the row's node.
 // PA_BOX_ASSEMBLY=0 keeps the general chunk kernel.
@@ -1014,10 +1029,19 @@ static int box_fast_setup(pa_handle h) {
   if (hx < 1 || hy < 1 || hz < 1 || (size_t)6 * hx * hy * hz != (size_t)h->ne) return PA_OK;
   const double step[3] = {h->box_len[0] / hx, h->box_len[1] / hy, h->box_len[2] / hz};
   const std::vector<int32_t>& gr = h->box_grid;
+  // Lattice positions must hold to the FP64 round-off of the coordinates themselves
+  // (a linspace axis of 1023 hexes deviates by ~1e-13 spacings): the general kernel's
+  // own edge vectors carry that relative error, so the lattice blocks then agree with it
+  // to rounding.
+  double tol[3];
+  for (int a = 0; a < 3; ++a) {
+    const double xmax = std::max(std::fabs(h->box_lo[a]), std::fabs(h->box_lo[a] + h->box_len[a]));
+    tol[a] = std::max(1e-13, 8.0 * DBL_EPSILON * xmax / step[a]);
+  }
   for (int n = 0; n < h->nn; ++n)
     for (int a = 0; a < 3; ++a) {
       const double t = (h->coords[3 * (size_t)n + a] - h->box_lo[a]) / step[a];
-      if (std::fabs(t - gr[3 * (size_t)n + a]) > 1e-13) return PA_OK;
+      if (!(std::fabs(t - gr[3 * (size_t)n + a]) <= tol[a])) return PA_OK;
     }
   // element types: sorted corner ids (bit 0 x, bit 1 y, bit 2 z) relative to the hex
   std::vector<std::array<uint8_t, 4>> types;
@@ -1654,6 +1678,7 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
   double rel = bb > 0 ? std::sqrt(rr0 / bb) : 0.0;
   if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
   bool conv = bb == 0.0 || rel <= opts.tolerance;
+  if (bb == 0.0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);  // b = 0: the solution is 0, whatever x0
   int it = 0, replacements = 0;
   double rr_ref = rr0;
   // z = V(r) and rz; kBetaInit also starts p = z, kBetaReplace recomputes beta for a
@@ -1811,6 +1836,26 @@ int pa_stress_pnorm(pa_handle h, const double* u, const double* z, int measure, 
   a.out_vec = dfdu ? dfdu : h->w_r.p;
   a.out_vec2 = dfdz ? dfdz : h->w_z.p;  // w_z holds >= num_nodes doubles
   PA_HIP(pa::launch_chunk(h->code_bits, pa::kModePNormGrad, a, h->stream));
+  return PA_OK;
+}
+
+int pa_stress_pnorm_gradient_x(pa_handle h, const double* u, const double* z, int measure, double pn, double q,
+                               double* dfdx, double* value) {
+  if (int rc = ready(h)) return rc;
+  if (!u || !z || !dfdx) return fail(PA_ERR_ARG, "pa_stress_pnorm_gradient_x: null pointer");
+  if (int rc = check_pnorm_args(measure, pn, q)) return rc;
+  double S = 0.0;
+  if (int rc = pnorm_value(h, u, z, measure, pn, q, &S)) return rc;
+  if (value) *value = std::pow(S, 1.0 / pn);
+  pa::ChunkArgs a = h->chunk_args();
+  a.u = u;
+  a.z = z;
+  a.measure = measure;
+  a.pn = pn;
+  a.q = q;
+  a.pscale = S > 0.0 ? std::pow(S, (1.0 - pn) / pn) / pn : 0.0;
+  a.out_vec = dfdx;
+  PA_HIP(pa::launch_chunk(h->code_bits, pa::kModePNormGradX, a, h->stream));
   return PA_OK;
 }
 

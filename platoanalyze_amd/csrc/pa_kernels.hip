@@ -96,12 +96,12 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   constexpr bool kRes = MODE == kModeResidual;
   constexpr bool kGzT = MODE == kModeGradZT;
   constexpr bool kGradZ = MODE == kModeEnergyGradZ;
-  constexpr bool kPnV = MODE == kModePNorm, kPnG = MODE == kModePNormGrad;
+  constexpr bool kPnV = MODE == kModePNorm, kPnG = MODE == kModePNormGrad, kPnX = MODE == kModePNormGradX;
   constexpr bool kAdj = MODE == kModeAdjZ;
   constexpr bool kGx = MODE == kModeGradX, kEGx = MODE == kModeEnergyGradX, kVol = MODE == kModeVolume;
   constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV || kVol;
   constexpr int kBlkW = kGzT ? 3 : (kGx ? 9 : 0);                      // doubles per block in LDS
-  constexpr int kRowW = (kRes || kPnG || kEGx || kVol) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row
+  constexpr int kRowW = (kRes || kPnG || kEGx || kVol || kPnX) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row
   constexpr int kRow2 = (kPnG || kVol) ? 1 : 0;                        // second per-row output
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
           if (ln[a] < nr) lds_add(&res[ln[a]], gz);
       }
     }
-    if constexpr (kPnV || kPnG) {
+    if constexpr (kPnV || kPnG || kPnX) {
       // measure m and, for the gradient, the stress-space direction t with
       // dm/du = c_u * B^T C0 t and dm/drho = c_z * m
       double mval, t[6], cu, cz;
@@ -211,6 +211,35 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
       const double mp = pow(mval, A.pn);
       if constexpr (kPnV) {
         if (ln[0] < nr) energy += mp * vol;
+      } else if constexpr (kPnX) {
+        // dS/dX_{b,k} = vol m^p g_bk (dvol/dX_{b,k} = vol g_bk)
+        //             + vol p m^(p-1) c_u (C0 t) . deps_{b,k},
+        // deps_{b,k}: Voigt strain (engineering shear) of dH/dX_{b,k} = -H[:,k] g_b^T,
+        // H = sum_a u_a g_a^T; rho does not depend on X.
+        double H[3][3];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int q2 = 0; q2 < 3; ++q2)
+            H[p][q2] = u[0][p] * g[0][q2] + u[1][p] * g[1][q2] + u[2][p] * g[2][q2] + u[3][p] * g[3][q2];
+        double tau[6];
+        iso_stress(A.m, t, tau);
+        const double mpm1 = mval > 0.0 ? mp / mval : 0.0;
+        const double fv = A.pscale * vol * mp;
+        const double fs = A.pscale * vol * A.pn * mpm1 * cu;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          if (ln[b] >= nr) continue;
+          double* dst = &res[3 * ln[b]];
+#pragma unroll
+          for (int k2 = 0; k2 < 3; ++k2) {
+            const double de = -(tau[0] * H[0][k2] * g[b][0] + tau[1] * H[1][k2] * g[b][1] + tau[2] * H[2][k2] * g[b][2] +
+                                tau[3] * (H[1][k2] * g[b][2] + H[2][k2] * g[b][1]) +
+                                tau[4] * (H[0][k2] * g[b][2] + H[2][k2] * g[b][0]) +
+                                tau[5] * (H[0][k2] * g[b][1] + H[1][k2] * g[b][0]));
+            lds_add(dst + k2, fv * g[b][k2] + fs * de);
+          }
+        }
       } else {
         // dS/du_a = vol p m^(p-1) c_u B_a^T C0 t ; dS/dz_a = vol p m^p c_z / 4
         const double mpm1 = mval > 0.0 ? mp / mval : 0.0;
@@ -710,7 +739,8 @@ void k_jac_chunk(ChunkArgs A) {
 size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
   if (mode == kModeJacobian || mode == kModeJacobianResidual) return jac_lds_layout(max_rows, max_blocks, max_nodes).total;
   const int blkw = mode == kModeGradZT ? 3 : (mode == kModeGradX ? 9 : 0);
-  const int roww = (mode == kModeResidual || mode == kModePNormGrad || mode == kModeEnergyGradX || mode == kModeVolume)
+  const int roww = (mode == kModeResidual || mode == kModePNormGrad || mode == kModeEnergyGradX || mode == kModeVolume ||
+                    mode == kModePNormGradX)
                        ? 3
                        : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
   const int row2 = (mode == kModePNormGrad || mode == kModeVolume) ? 1 : 0;
@@ -743,6 +773,7 @@ static hipError_t launch_chunk_b(int mode, const ChunkArgs& a, hipStream_t s) {
     case kModeGradX: k_chunk<BITS, kModeGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeEnergyGradX: k_chunk<BITS, kModeEnergyGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     case kModeVolume: k_chunk<BITS, kModeVolume, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
+    case kModePNormGradX: k_chunk<BITS, kModePNormGradX, kNtVec><<<grid, kNtVec, lds, s>>>(a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

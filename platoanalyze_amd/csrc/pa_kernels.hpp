@@ -114,7 +114,8 @@ struct ChunkArgs {
 
 enum ChunkMode { kModeJacobian = 0, kModeJacobianResidual = 1, kModeResidual = 2, kModeGradZT = 3,
                  kModeEnergy = 4, kModeEnergyGradZ = 5, kModePNorm = 6, kModePNormGrad = 7, kModeAdjZ = 8,
-                 kModeGradX = 9, kModeEnergyGradX = 10, kModeVolume = 11, kNumChunkModes = 12 };
+                 kModeGradX = 9, kModeEnergyGradX = 10, kModeVolume = 11, kModePNormGradX = 12,
+                 kNumChunkModes = 13 };
 
 
 // launch helpers (defined in pa_kernels.hip)

@@ -43,11 +43,9 @@ import torch
 
 from .elliptic import ElasticModel, EssentialBC, PhysicsScalarFunction, Problem, Traction, Volume, VectorFunction
 from .elliptic import apply_constraints, solve
-from ._capi import PaError
 from .exodus import ExodusMesh
 from .mesh import infer_box  # noqa: F401  (part of this module's interface)
 
-PA_ERR_MESH = 3  # include/plato_mi355x.h
 
 _MISSING = object()
 
@@ -275,20 +273,15 @@ class ElasticProblem:
     def _handle(self, penalty, with_box: bool = False) -> ElasticModel:
         typ, p, rmin, length = penalty
 
-        def make(box):
-            return ElasticModel(self.mesh.mesh, self.E, self.nu, p, rmin, self.bcs, self.tractions,
-                                device=self.device, box=box, penalty_model=typ, regularization_length=length)
-
-        if not (with_box and self.box):
-            return make(None)
-        try:
-            return make(self.box)
-        except PaError as e:
-            # a tensor lattice with unequal spacings is no box: only the multigrid solve needs one
-            if self._pre == "multigrid" or e.status != PA_ERR_MESH:
-                raise
+        box = self.box if with_box else None
+        # a tensor lattice with unequal spacings is no box: only the multigrid solve needs
+        # one, otherwise the same handle goes on with the general assembly
+        m = ElasticModel(self.mesh.mesh, self.E, self.nu, p, rmin, self.bcs, self.tractions,
+                         device=self.device, box=box, penalty_model=typ, regularization_length=length,
+                         box_optional=self._pre != "multigrid")
+        if box is not None and m.box is None:
             self.box = None
-            return make(None)
+        return m
 
     def _criteria(self, pp: ParameterList) -> dict:
         out = {}
@@ -351,11 +344,8 @@ class ElasticProblem:
         if c.kind == "Energy":
             f = PhysicsScalarFunction(c.model)
             return f.gradient_u(u, z), (f.gradient_z(u, z) if wrt == "z" else f.gradient_x(u, z))
-        if wrt == "x":
-            raise NotImplementedError(f"criterion {c.name!r}: the configuration gradient of the stress p-norm "
-                                      "is not implemented")
         f = self._pnorm(c)
-        return f.gradient_u(u, z), f.gradient_z(u, z)
+        return f.gradient_u(u, z), (f.gradient_z(u, z) if wrt == "z" else f.gradient_x(u, z))
 
     def _adjoint(self, u, z, dfdu: torch.Tensor, bc_scale: float) -> torch.Tensor:
         """lambda with K(z) lambda = -dF/du, the residual's K constrained as the reference

@@ -27,6 +27,18 @@ def _dptr(t: torch.Tensor):
     return C.c_void_p(t.data_ptr())
 
 
+def _dvec(t, n: int, what: str):
+    """Device pointer of a contiguous float64 vector of exactly n entries; ValueError
+    otherwise (the library reads or writes n doubles through the pointer)."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{what}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{what}: expected float64, got {t.dtype}")
+    if t.numel() != n:
+        raise ValueError(f"{what}: expected {n} entries, got {t.numel()}")
+    return _dptr(t)
+
+
 def _hptr(a: np.ndarray):
     return C.c_void_p(a.ctypes.data)
 
@@ -92,6 +104,7 @@ class ElasticModel:
     box: tuple | None = None  # (nx, ny, nz) hexes of a structured box mesh: enables multigrid
     penalty_model: str = "SIMP"  # "Penalty Function" Type: SIMP, RAMP, Heaviside, NoPenalty
     regularization_length: float = 1.0  # Heaviside only
+    box_optional: bool = False  # a box the library refuses (nodes off the lattice) leaves box = None
 
     def __post_init__(self):
         h = C.c_void_p()
@@ -116,7 +129,13 @@ class ElasticModel:
             _capi.call("pa_set_graph", h, _hptr(rm), _hptr(ci))
         _capi.call("pa_finalize", h)
         if self.box is not None:
-            _capi.call("pa_set_box", h, *(int(v) for v in self.box))
+            try:
+                _capi.call("pa_set_box", h, *(int(v) for v in self.box))
+            except _capi.PaError as e:
+                # the finalized handle stays usable without a box (general assembly)
+                if not (self.box_optional and e.status == _capi.PA_ERR_MESH):
+                    raise
+                self.box = None
         nr, nb = C.c_int(), C.c_int()
         _capi.call("pa_graph_size", h, C.byref(nr), C.byref(nb))
         self.num_rows, self.nnz_blocks = nr.value, nb.value
@@ -140,6 +159,10 @@ class ElasticModel:
     def _stream(self):
         _capi.call("pa_set_stream", self._h, C.c_void_p(torch.cuda.current_stream().cuda_stream))
 
+    def _uz(self, u, z):
+        """Checked device pointers of a state (3N float64) and a control (N float64)."""
+        return _dvec(u, self.num_dofs, "state u"), _dvec(z, self.num_nodes, "control z")
+
     def close(self):
         if getattr(self, "_h", None):
             _capi.lib().pa_destroy(self._h)
@@ -159,10 +182,9 @@ class VectorFunction:
         self.m = model
 
     def _check(self, u, z):
-        if u is not None and (u.dtype != torch.float64 or u.numel() != self.m.num_dofs):
-            raise ValueError("state must be float64 with 3*num_nodes entries")
-        if z.dtype != torch.float64 or z.numel() != self.m.num_nodes:
-            raise ValueError("control must be float64 with num_nodes entries")
+        if u is not None:
+            _dvec(u, self.m.num_dofs, "state u")
+        _dvec(z, self.m.num_nodes, "control z")
 
     def value(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
         self._check(u, z)
@@ -177,7 +199,7 @@ class VectorFunction:
         vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
         self.m._stream()
         _capi.call("pa_vf_gradient_u", self.m.handle, _dptr(u) if u is not None else None, _dptr(z), _dptr(vals),
-                   _dptr(residual_out) if residual_out is not None else None)
+                   _dvec(residual_out, self.m.num_dofs, "residual_out") if residual_out is not None else None)
         return CrsMatrix(self.m.row_map, self.m.column_indices, vals, 3, 3)
 
     gradient_u_T = gradient_u  # the elastostatic Jacobian is symmetric
@@ -208,27 +230,31 @@ class PhysicsScalarFunction:
 
     def value(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> float:
         v = C.c_double()
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_sf_value", self.m.handle, _dptr(u), _dptr(z), C.byref(v))
+        _capi.call("pa_sf_value", self.m.handle, up, zp, C.byref(v))
         return v.value
 
     def gradient_u(self, u: torch.Tensor, z: torch.Tensor, step: int = 0, time_step: float = 0.0) -> torch.Tensor:
         g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_sf_gradient_u", self.m.handle, _dptr(u), _dptr(z), _dptr(g))
+        _capi.call("pa_sf_gradient_u", self.m.handle, up, zp, _dptr(g))
         return g
 
     def gradient_z(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
         g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_sf_gradient_z", self.m.handle, _dptr(u), _dptr(z), _dptr(g), None)
+        _capi.call("pa_sf_gradient_z", self.m.handle, up, zp, _dptr(g), None)
         return g
 
     def gradient_x(self, u: torch.Tensor, z: torch.Tensor, time_step: float = 0.0) -> torch.Tensor:
         """Partial derivative of the energy in the nodal coordinates (3N)."""
         g = torch.empty(self.m.num_dofs, dtype=torch.float64, device=u.device)
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_sf_gradient_x", self.m.handle, _dptr(u), _dptr(z), _dptr(g))
+        _capi.call("pa_sf_gradient_x", self.m.handle, up, zp, _dptr(g))
         return g
 
 
@@ -245,10 +271,11 @@ class Volume:
 
     def _eval(self, z: torch.Tensor, want_z: bool, want_x: bool):
         v = C.c_double()
+        zp = _dvec(z, self.m.num_nodes, "control z")
         gz = torch.empty(self.m.num_nodes, dtype=torch.float64, device=z.device) if want_z else None
         gx = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device) if want_x else None
         self.m._stream()
-        _capi.call("pa_volume", self.m.handle, _dptr(z), *self.args, C.byref(v),
+        _capi.call("pa_volume", self.m.handle, zp, *self.args, C.byref(v),
                    _dptr(gz) if gz is not None else None, _dptr(gx) if gx is not None else None)
         return v.value, gz, gx
 
@@ -287,8 +314,11 @@ class Problem:
     def solution(self, z: torch.Tensor, u0: torch.Tensor | None = None) -> torch.Tensor:
         """Problem::solution; `u0` warm-starts the solve (any precision and preconditioner;
         its constrained dofs are reset to their prescribed values)."""
+        zp = _dvec(z, self.m.num_nodes, "control z")
         if u0 is not None:
-            u = u0.detach().clone().contiguous()
+            if u0.numel() != self.m.num_dofs:
+                raise ValueError(f"u0: expected {self.m.num_dofs} entries, got {u0.numel()}")
+            u = u0.detach().to(device=z.device, dtype=torch.float64).clone().contiguous()
             self.opts.use_x0 = 1
         else:
             u = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device)
@@ -299,7 +329,7 @@ class Problem:
                 self._vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
             vals = _dptr(self._vals)
         self.m._stream()
-        _capi.call("pa_solution", self.m.handle, _dptr(z), _dptr(u), vals, C.byref(self.opts), C.byref(self.stats))
+        _capi.call("pa_solution", self.m.handle, zp, _dptr(u), vals, C.byref(self.opts), C.byref(self.stats))
         self.state = u
         return u
 
@@ -312,22 +342,28 @@ class Problem:
             u = self.solution(z)
         g = torch.empty(self.m.num_nodes, dtype=torch.float64, device=z.device)
         v = C.c_double()
+        up, zp = self.m._uz(u, z)
         self.m._stream()
-        _capi.call("pa_compliance_gradient", self.m.handle, _dptr(u), _dptr(z), _dptr(g), C.byref(v))
+        _capi.call("pa_compliance_gradient", self.m.handle, up, zp, _dptr(g), C.byref(v))
         return g, v.value
 
 
 def solve(model: ElasticModel, values: torch.Tensor, b: torch.Tensor, tolerance: float = 1e-10,
           max_iters: int = 200000, x0: torch.Tensor | None = None):
     """AbstractSolver::solve(A, x, b) on a constrained matrix of the model's graph."""
+    vp = _dvec(values, 9 * model.nnz_blocks, "values")
+    bp = _dvec(b, model.num_dofs, "b")
+    if x0 is not None:
+        _dvec(x0, model.num_dofs, "x0")
     x = x0.clone() if x0 is not None else torch.empty_like(b)
     opts = _capi.SolverOpts(tolerance, max_iters, 16, 0, 1 if x0 is not None else 0, 1e-4, 0)
     st = _capi.SolverStats()
     model._stream()
-    _capi.call("pa_solve", model.handle, _dptr(values), _dptr(x), _dptr(b), C.byref(opts), C.byref(st))
+    _capi.call("pa_solve", model.handle, vp, _dptr(x), bp, C.byref(opts), C.byref(st))
     return x, st
 
 
 def apply_constraints(model: ElasticModel, values: torch.Tensor, rhs: torch.Tensor, scale: float = 1.0):
+    vp, rp = _dvec(values, 9 * model.nnz_blocks, "values"), _dvec(rhs, model.num_dofs, "rhs")
     model._stream()
-    _capi.call("pa_apply_constraints", model.handle, _dptr(values), _dptr(rhs), float(scale))
+    _capi.call("pa_apply_constraints", model.handle, vp, rp, float(scale))

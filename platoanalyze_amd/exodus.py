@@ -211,7 +211,6 @@ def model_from_exodus(ex: ExodusMesh, E: float = 1.0, nu: float = 0.3, penalty_e
     tractions = {side set: (tx, ty, tz)}.  A mesh whose nodes form a uniform box lattice
     is declared a box (lattice assembly, multigrid available) unless `box` is given;
     box=None declares none."""
-    from ._capi import PaError
     from .elliptic import ElasticModel, EssentialBC, Traction
     from .mesh import infer_box
 
@@ -230,11 +229,6 @@ def model_from_exodus(ex: ExodusMesh, E: float = 1.0, nu: float = 0.3, penalty_e
     box = kw.pop("box", "auto")
     if box != "auto":
         return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, box=box, **kw)
-    box = infer_box(ex.mesh)
-    if box is not None:
-        try:
-            return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, box=box, **kw)
-        except PaError as e:
-            if e.status != 3:  # PA_ERR_MESH: a tensor lattice with unequal spacings is no box
-                raise
-    return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, **kw)
+    # a tensor lattice with unequal spacings is no box: the handle then keeps the general assembly
+    return ElasticModel(ex.mesh, E, nu, penalty_exponent, penalty_minimum, bcs, trs, box=infer_box(ex.mesh),
+                        box_optional=True, **kw)

@@ -26,7 +26,7 @@ import torch
 
 from . import _capi
 from . import workloads as wl
-from .elliptic import Problem, _dptr
+from .elliptic import Problem
 from .unet import IN_CH, OUT_CH, UNet
 
 
@@ -155,7 +155,8 @@ def energy(model, z: torch.Tensor, u: torch.Tensor, f: torch.Tensor) -> float:
 def smooth(model, z: torch.Tensor, u: torch.Tensor, steps: int, omega: float = 0.6) -> None:
     """In place: `steps` damped block-Jacobi sweeps of the homogeneous problem (pa_smooth)."""
     model._stream()
-    _capi.call("pa_smooth", model.handle, _dptr(z), _dptr(u), int(steps), float(omega))
+    up, zp = model._uz(u, z)
+    _capi.call("pa_smooth", model.handle, zp, up, int(steps), float(omega))
 
 
 class WarmStart:

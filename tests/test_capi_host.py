@@ -151,3 +151,90 @@ def test_set_box_accepts_lattices_in_any_node_order():
             assert L.pa_set_box(h, 4, 2, 3) == 3, bad
         finally:
             L.pa_destroy(h)
+
+
+class _NoLibraryModel:
+    """Stands in for an ElasticModel on the CPU: sizes only; any library call fails."""
+    num_nodes, num_dofs, nnz_blocks = 27, 81, 343
+    handle = None
+
+    def _stream(self):
+        raise AssertionError("library reached before the argument checks")
+
+    def _uz(self, u, z):
+        from platoanalyze_amd.elliptic import ElasticModel
+        return ElasticModel._uz(self, u, z)
+
+
+@pytest.mark.parametrize("bad", ["float32", "short", "long"])
+def test_device_arguments_checked_before_library_calls(bad):
+    """float32 tensors (torch's default dtype) and vectors of the wrong length are refused
+    with ValueError before any library call: the kernels read and write n doubles through
+    the pointers, so either would be an out-of-bounds device access."""
+    import torch
+
+    from platoanalyze_amd import criteria
+    from platoanalyze_amd.elliptic import (PhysicsScalarFunction, Problem, VectorFunction, Volume,
+                                           apply_constraints, solve)
+
+    m = _NoLibraryModel()
+
+    def vec(n):
+        if bad == "float32":
+            return torch.zeros(n, dtype=torch.float32)
+        return torch.zeros(n + (-1 if bad == "short" else 1), dtype=torch.float64)
+
+    good_u, good_z = torch.zeros(81, dtype=torch.float64), torch.zeros(27, dtype=torch.float64)
+    calls = [
+        lambda: VectorFunction(m).value(vec(81), good_z),
+        lambda: VectorFunction(m).gradient_u(good_u, vec(27)),
+        lambda: PhysicsScalarFunction(m).value(vec(81), good_z),
+        lambda: PhysicsScalarFunction(m).gradient_z(good_u, vec(27)),
+        lambda: PhysicsScalarFunction(m).gradient_x(vec(81), good_z),
+        lambda: Volume(m).value(vec(27)),
+        lambda: Problem(m).solution(vec(27)),
+        lambda: Problem(m).criterionGradient(good_z, vec(81)),
+        lambda: criteria.StressPNorm(m).value(vec(81), good_z),
+        lambda: criteria.StressPNorm(m).gradient_x(good_u, vec(27)),
+        lambda: criteria.dRdz_T_apply(m, good_u, vec(81), good_z),
+        lambda: solve(m, vec(9 * 343), good_u),
+        lambda: apply_constraints(m, torch.zeros(9 * 343, dtype=torch.float64), vec(81)),
+    ]
+    if bad != "float32":  # a float32 warm start of the right length is converted, not refused
+        calls.append(lambda: Problem(m).solution(good_z, u0=vec(81)))
+    for k, call in enumerate(calls):
+        with pytest.raises(ValueError):
+            call()
+
+
+def test_set_box_coordinate_roundoff_and_degenerate_axes():
+    """Lattices stored in single precision (offsets ~1e-5 spacings at 128 hexes on a
+    0.3-long box) are still boxes; a flat axis (zero extent, whose node count can still
+    match) and non-finite coordinates are refused with PA_ERR_MESH, not indexed."""
+    from platoanalyze_amd.mesh import TetMesh
+
+    L = _capi.lib()
+    m = build_box(128, 2, 2, 0.3, 0.01, 0.01)
+    c32 = m.coords.astype(np.float32).astype(np.float64)
+    t = c32[:, 0] / (0.3 / 128)
+    assert np.abs(t - np.round(t)).max() > 1e-6  # off the FP64 lattice by more than the old bound
+    h = _handle(TetMesh(c32, m.conn))
+    try:
+        assert L.pa_set_box(h, 128, 2, 2) == 0
+    finally:
+        L.pa_destroy(h)
+    flat = build_box(1, 1, 3, 1.0, 1.0, 1.0)
+    c = flat.coords.copy()
+    c[:, 1] = 0.0  # (2)(2)(4) nodes, all on y = 0: node count matches nx=1, ny=1, nz=3
+    h = _handle(TetMesh(c, flat.conn))
+    try:
+        assert L.pa_set_box(h, 1, 1, 3) == 3
+    finally:
+        L.pa_destroy(h)
+    c = flat.coords.copy()
+    c[3, 2] = np.nan
+    h = _handle(TetMesh(c, flat.conn))
+    try:
+        assert L.pa_set_box(h, 1, 1, 3) == 3
+    finally:
+        L.pa_destroy(h)

@@ -268,6 +268,15 @@ def _oracle_compliance(coords, w, z, rm, ci):
     return o.energy(u, z, False, False)[0]
 
 
+def _oracle_pnorm(coords, w, z, rm, ci, pn):
+    """Voigt stress p-norm (SIMP p = 3, minimum 1e-3) of the oracle's dense solve."""
+    o = orc.Model(coords, w.mesh.conn, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min, w.face_verts, w.traction)
+    f = -o.residual(np.zeros(3 * w.mesh.num_nodes), z)
+    Kc, bc = orc.apply_constraints(rm, ci, o.jacobian(None, z, rm, ci), f, w.bc_dofs, np.zeros(len(w.bc_dofs)))
+    u = np.linalg.solve(orc.to_dense(rm, ci, Kc), bc)
+    return o.pnorm(u, z, 0, pn, 0.0, False, False)[0]
+
+
 def test_gpu_criterion_gradients_non_self_adjoint():
     """Non-self-adjoint dC/dz equals the self-adjoint -dC/dz|_u (lambda = -u under zero
     Dirichlet values); non-self-adjoint dC/dX equals central differences (h = 1e-6) of the
@@ -330,7 +339,9 @@ def test_gpu_pnorm_generic_adjoint_matches_fused():
     (pa_stress_pnorm_gradient); forced onto the generic path (assembled K, adjoint
     constraints, PCG, (dR/dz)^T lambda by the block product) it gives the same gradient.
     A criterion with its own penalty (RAMP) is evaluated on a second handle, checked
-    against central differences of its value over two controls."""
+    against central differences of its value over two controls.  The configuration
+    gradient of the p-norm (adjoint with (dR/dX)^T lambda) against central differences of
+    the oracle's p-norm over nodal coordinates."""
     print("case: deck p-norm adjoints", flush=True)
     w, ex, z, _ = _cantilever_mesh((4, 2, 2))
     dz = dev(z)
@@ -353,8 +364,19 @@ def test_gpu_pnorm_generic_adjoint_matches_fused():
             vals.append(pb.criterionValue(d, pb.solution(d), "Stress RAMP"))
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g[n]) <= 1e-6 * np.abs(g).max() + 1e-5 * abs(fd), (n, fd, g[n])
-    with pytest.raises(NotImplementedError):
-        pb.criterionGradientX(dz, u, "Stress")
+    # dF/dX of the p-norm through the adjoint: central differences (h = 1e-6) of the
+    # oracle's p-norm of dense solves at displaced coordinates
+    gx = host(pb.criterionGradientX(dz, u, "Stress"))
+    rm, ci = host(pb.model.row_map), host(pb.model.column_indices)
+    for n in (int(np.argmax(np.abs(gx)) // 3), w.mesh.num_nodes // 2, int(w.face_verts[0, 0])):
+        for k in range(3):
+            vals = []
+            for s in (1.0, -1.0):
+                X = w.mesh.coords.copy()
+                X[n, k] += s * h
+                vals.append(_oracle_pnorm(X, w, z, rm, ci, 6.0))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - gx[3 * n + k]) <= 1e-6 * np.abs(gx).max(), (n, k, fd, gx[3 * n + k])
 
 
 APP = """

@@ -416,3 +416,155 @@ def test_penalty_models_fd_and_limits(small, pm):
     if kind == 2:  # saturates above the regularization length
         np.testing.assert_allclose(mod.residual(u, ones * L * 1.01), simp1.residual(u, ones), rtol=0,
                                    atol=1e-13 * np.abs(R0).max())
+
+
+# ----------------------------------------------------------------- further reference goldens
+def test_von_mises_golden(goldens):
+    """VonMisesYieldFunction<3> on the two cells of AugLag_VonMises3D
+    (PlatoAugLagStressTest.cpp:1063-1101), at the reference's own 1e-4 tolerance (its
+    inputs and golds are printed with six decimals)."""
+    g = goldens["von_mises3d"]
+    got = orc.von_mises(np.array(g["stress"]))
+    for v, r in zip(got, g["gold"]):
+        assert abs(v - r) <= g["tolerance"] * abs(r), (v, r)
+    # a wrong shear factor (1 instead of 3) or a missing 1/2 fails the golden
+    s = np.array(g["stress"])
+    d = s[:, 0:3] - np.roll(s[:, 0:3], -1, axis=1)
+    bad = np.sqrt(0.5 * (d ** 2).sum(1) + 1.0 * (s[:, 3:] ** 2).sum(1))
+    assert np.all(np.abs(bad - g["gold"]) > 1e-4 * np.abs(g["gold"]))
+
+
+def test_volume_golden(box222, goldens):
+    """Geometric Volume3D (DerivativeTests.cpp:2151-2302): SIMP exponent 1, minimum 0,
+    z = 1 on the 2x2x2 box: value 1, dV/dz (27) and dV/dX (81) at 1e-13."""
+    g = goldens["volume3d"]
+    model = orc.Model(box222.coords, box222.conn)
+    v, gz, gx = model.volume(np.ones(27), model=0, p=1.0, rmin=0.0)
+    assert abs(v - g["value_gold"]) <= 1e-13 * g["value_gold"]
+    assert_close(gz, g["grad_z_gold"], 1e-13)
+    assert_close(gx, g["grad_x_gold"], 1e-13)
+
+
+def test_stress_pnorm_gradient_x_golden(ref_model, goldens):
+    """StressPNorm3D gradient_x (DerivativeTests.cpp:1113-1137): 27 values at p = 12,
+    1e-13 as the reference (1e-12 where the gold is printed short)."""
+    F, gx = ref_model.pnorm_gradient_x(ref_state(27), np.ones(27), 0, 12.0)
+    assert abs(F - goldens["stress_pnorm3d"]["value_gold"]) <= 1e-13 * F
+    assert_close(gx[:27], goldens["stress_pnorm3d"]["grad_x_gold"], 1e-12)
+
+
+def test_pnorm_gradient_x_fd(small):
+    """dF/dX of both p-norm measures against central differences of the value in the
+    nodal coordinates (interior and boundary nodes)."""
+    m, z, u, model, rm, ci = small
+    for measure, pn, q in [(0, 8.0, 0.0), (1, 8.0, 0.5)]:
+        F, gx = model.pnorm_gradient_x(u, z, measure, pn, q)
+        for n in [0, 7, m.num_nodes // 2, m.num_nodes - 1]:
+            for k in range(3):
+                h = 1e-6
+                Xp, Xm = m.coords.copy(), m.coords.copy()
+                Xp[n, k] += h
+                Xm[n, k] -= h
+                mp = orc.Model(Xp, m.conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3)
+                mm = orc.Model(Xm, m.conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3)
+                fd = (mp.pnorm(u, z, measure, pn, q, False, False)[0]
+                      - mm.pnorm(u, z, measure, pn, q, False, False)[0]) / (2 * h)
+                assert abs(fd - gx[3 * n + k]) <= 1e-6 * np.abs(gx).max(), (measure, n, k, fd, gx[3 * n + k])
+
+
+# ----------------------------------------------------------------- penalty weights, closed forms
+# Hand-computed weights at the interior density c = 0.5 (and their slopes), from the
+# reference's formulas (src/Simp.hpp:130-134, src/Ramp.hpp:94-100, src/Heaviside.hpp:110-125):
+#   SIMP p=3, rmin=1e-3:  w = 1e-3 + 0.999 * 0.5^3 = 0.125875;  w' = 0.999 * 3 * 0.25 = 0.74925
+#   RAMP p=3, rmin=1e-3:  0.5 / (1 + 3 * 0.5) = 0.2  ->  w = 0.2008;  w' = 0.999 * 4 / 2.5^2 = 0.63936
+#   Heaviside p=2, rmin=1e-3, L=1.2: pi * 0.5 / (2 * 1.2) = 37.5 degrees,
+#       h = (1 + sin 37.5deg) / 2,  w = 1e-3 + 0.999 h^2,  w' = 0.999 * 2 h * cos(37.5deg) / 2 * pi / 2.4
+#   none: w = 1, w' = 0
+_S37 = np.sin(np.radians(37.5))
+_C37 = np.cos(np.radians(37.5))
+_H37 = 0.5 * (1.0 + _S37)
+WEIGHTS_AT_HALF = [  # (model, p, rmin, L, w(0.5), w'(0.5))
+    (0, 3.0, 1e-3, 1.0, 0.125875, 0.74925),
+    (1, 3.0, 1e-3, 1.0, 0.2008, 0.63936),
+    (2, 2.0, 1e-3, 1.2, 1e-3 + 0.999 * _H37 ** 2, 0.999 * 2.0 * _H37 * 0.5 * _C37 * np.pi / 2.4),
+    (3, 1.0, 0.0, 1.0, 1.0, 0.0),
+]
+
+
+@pytest.mark.parametrize("kind,p,rmin,L,w,dw", WEIGHTS_AT_HALF)
+def test_penalty_weight_closed_form_uniform(small, kind, p, rmin, L, w, dw):
+    """Uniform z = 0.5 against the unit-weight problem (penalty "none", K1): K(z) = w K1,
+    R(u,z) + f = w (R1(u) + f), C(z) = w C1, sum_n dC/dz_n = w' C1, dR/dz summed over
+    the control nodes = w' K1 u, the Voigt p-norm scales by w and the volume by w.  A
+    dropped (1 - rmin) factor, a wrong exponent or a weight of the mean replaced by the
+    mean of the weights fails here."""
+    m, z, u, _, rm, ci = small
+    fv, fe = faces_on_plane(m, 0, 2.0)
+    t = (0.0, 0.0, -1.0)
+    mod = orc.Model(m.coords, m.conn, E=1.0, nu=0.3, p=p, rmin=rmin, face_verts=fv, traction=t,
+                    penalty_model=kind, length=L)
+    unit = orc.Model(m.coords, m.conn, E=1.0, nu=0.3, p=1.0, rmin=0.0, face_verts=fv, traction=t,
+                     penalty_model=3)
+    half, one = np.full(m.num_nodes, 0.5), np.ones(m.num_nodes)
+    K1 = unit.jacobian(u, one, rm, ci)
+    Kc = mod.jacobian(u, half, rm, ci)
+    np.testing.assert_allclose(Kc, w * K1, rtol=0, atol=1e-13 * np.abs(K1).max())
+    f = -mod.residual(np.zeros_like(u), one)
+    np.testing.assert_allclose(mod.residual(u, half) + f, w * (unit.residual(u, one) + f), rtol=0,
+                               atol=1e-13 * np.abs(K1).max() * np.abs(u).max())
+    C1 = unit.energy(u, one, False, False)[0]
+    Cc, _, gz = mod.energy(u, half, False, True)
+    assert abs(Cc - w * C1) <= 1e-13 * C1
+    assert abs(gz.sum() - dw * C1) <= 1e-12 * max(abs(dw) * C1, 1e-300) + 1e-15 * C1
+    G = mod.gradient_z(u, half, rm, ci)  # rows = control node; entries dR_(col,i)/dz_row
+    colsum = np.zeros(3 * m.num_nodes)
+    for r in range(m.num_nodes):
+        for s in range(rm[r], rm[r + 1]):
+            colsum[3 * ci[s]:3 * ci[s] + 3] += G[3 * s:3 * s + 3]
+    K1u = orc.to_dense(rm, ci, K1) @ u
+    np.testing.assert_allclose(colsum, dw * K1u, rtol=0, atol=1e-12 * np.abs(K1u).max())
+    F1 = unit.pnorm(u, one, 0, 8.0, 0.0, False, False)[0]
+    Fc = mod.pnorm(u, half, 0, 8.0, 0.0, False, False)[0]
+    assert abs(Fc - w * F1) <= 1e-13 * F1
+    box = 2.0
+    assert abs(mod.volume(half, kind, p, rmin, L)[0] - w * box) <= 1e-13 * box
+
+
+@pytest.mark.parametrize("kind,p,rmin,L", [(0, 3.0, 1e-3, 1.0), (1, 3.0, 1e-3, 1.0), (2, 2.0, 1e-3, 1.2)])
+def test_penalty_weight_of_mean_density_one_element(kind, p, rmin, L):
+    """One tet with distinct nodal densities (mean 0.525): its stiffness is
+    w(mean z) K(z = 1) (ApplyWeighting averages the four nodal controls before the
+    penalty, src/ApplyWeighting.hpp:49-65)."""
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    conn = np.array([[0, 1, 2, 3]], np.int32)
+    rm, ci = orc.graph(conn, 4)
+    mod = orc.Model(X, conn, E=1.0, nu=0.3, p=p, rmin=rmin, penalty_model=kind, length=L)
+    zr = np.array([0.2, 0.4, 0.6, 0.9])
+    rho = 0.525
+    if kind == 0:
+        w = 1e-3 + 0.999 * rho ** 3
+    elif kind == 1:
+        w = 1e-3 + 0.999 * rho / (1.0 + 3.0 * (1.0 - rho))
+    else:
+        w = 1e-3 + 0.999 * (0.5 * (1.0 + np.sin(np.pi * rho / 2.4))) ** 2
+    K1 = orc.Model(X, conn, E=1.0, nu=0.3, p=1.0, rmin=0.0, penalty_model=3).jacobian(None, np.ones(4), rm, ci)
+    np.testing.assert_allclose(mod.jacobian(None, zr, rm, ci), w * K1, rtol=0, atol=1e-14 * np.abs(K1).max())
+
+
+def test_von_mises_pnorm_density_scaling(small):
+    """Relaxed von Mises p-norm: uniform z = c scales F by c^q exactly (0.5^0.5), and
+    the measure of one element equals rho^q times the von Mises stress of its
+    unpenalized stress (checked through the p = 1 norm of a one-element mesh)."""
+    m, z, u, model, rm, ci = small
+    F1 = model.pnorm(u, np.ones(m.num_nodes), 1, 8.0, 0.5, False, False)[0]
+    Fc = model.pnorm(u, np.full(m.num_nodes, 0.5), 1, 8.0, 0.5, False, False)[0]
+    assert abs(Fc - np.sqrt(0.5) * F1) <= 1e-13 * F1
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    conn = np.array([[0, 1, 2, 3]], np.int32)
+    one = orc.Model(X, conn, E=1.0, nu=0.3, p=3.0, rmin=1e-3)
+    ue = np.random.default_rng(3).normal(size=12) * 1e-2
+    zr = np.array([0.2, 0.4, 0.6, 0.9])
+    _, vol, eps, sig, _ = one.cell_worksets(ue, 1)
+    vm = orc.von_mises(sig)[0]
+    F = one.pnorm(ue, zr, 1, 1.0, 0.5, False, False)[0]
+    assert abs(F - vol[0] * 0.525 ** 0.5 * vm) <= 1e-14 * F
