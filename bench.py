@@ -1,8 +1,8 @@
 """Benchmark: Tet4 SIMP Jacobian + residual assembly throughput on one MI355X.
 
 A step is one full assembly (VectorFunction::gradient_u with the fused residual
-R = K(z)u - f) of the 128x64x64 x 6 Tet4 cantilever (3.15M tets, 545k nodes), inputs
-resident in HBM.  Prints ONE JSON line.  Also measured (reported beside `value`):
+R = K(z)u - f) of the 256x128x128 x 6 Tet4 cantilever (25.2M tets, 4.28M nodes, 63.5M
+graph blocks), inputs resident in HBM.  Prints ONE JSON line.  Also measured (reported beside `value`):
 solve + sensitivity time (Problem::solution to 1e-8 relative true residual, then
 dC/dz) for FP64 and mixed-precision PCG with block-Jacobi and multigrid
 preconditioning, and the CPU baseline (the oracle restatement of the reference
@@ -33,36 +33,81 @@ def asm_bytes(E: int, N: int, nnzb: int) -> int:
     return 16 * E + 56 * N + 4 * (N + 1) + 4 * nnzb + 72 * nnzb + 24 * N
 
 
-def cpu_baseline(threads: int) -> dict:
-    """Oracle (reference algorithm restated: SFad worksets, atomic scatter after a row
-    search, two Jacobian passes) on the 64x32x32 x 6 Tet4 sub-box of the same physics."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def slab_sample(w, nxs: int):
+    """The first `nxs` hex layers along x of a box workload: same lattice spacing, element
+    shapes, material and densities (the workload's own density field restricted to the
+    slab), as a bounded CPU sample whose per-element work is that of the full mesh."""
+    from platoanalyze_amd.mesh import build_box, density_lattice, lattice_to_nodes
+
+    c = w.cfg
+    h = (c.lx / c.nx, c.ly / c.ny, c.lz / c.nz)
+    mesh = build_box(nxs, c.ny, c.nz, nxs * h[0], c.ly, c.lz)
+    lat = density_lattice(c.nx, c.ny, c.nz, c.seed, rho_min=c.rho_min)
+    return mesh, lattice_to_nodes(lat, mesh.coords, *h)
+
+
+def cpu_baseline(w, threads: int) -> dict:
+    """The oracle (the reference algorithm restated: SFad worksets, atomic scatter after a
+    linear row search, two Jacobian passes) timed on slabs of the benched workload:
+    gradient_u + value on a 32-hex slab with `threads` OpenMP threads (median of 5 after
+    one warm-up) and on a 4-hex slab with one thread (median of 3)."""
     import numpy as np
 
     from oracle import oracle as orc
-    from platoanalyze_amd import workloads as wl
 
-    w = wl.make("cantilever_64x32x32")
-    o = orc.Model(w.mesh.coords, w.mesh.conn, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min, w.face_verts,
-                  w.traction)
-    rm, ci = orc.graph(w.mesh.conn, w.mesh.num_nodes)
-    u = np.random.default_rng(1).normal(size=3 * w.mesh.num_nodes) * 1e-3
-    def timed(nthreads: int, reps: int) -> float:
+    isa = orc.select_isa()
+
+    def timed(nxs: int, nthreads: int, reps: int):
+        mesh, z = slab_sample(w, nxs)
+        o = orc.Model(mesh.coords, mesh.conn, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min)
+        rm, ci = orc.graph(mesh.conn, mesh.num_nodes)
+        u = np.random.default_rng(1).normal(size=3 * mesh.num_nodes) * 1e-3
         orc.set_threads(nthreads)
         times = []
-        for rep in range(reps + 1):  # 1 warm-up
+        for _ in range(reps + 1):  # 1 warm-up
             t = time.perf_counter()
-            o.jacobian(u, w.z, rm, ci)
-            o.residual(u, w.z)
+            o.jacobian(u, z, rm, ci)
+            o.residual(u, z)
             times.append(time.perf_counter() - t)
-        return sorted(times[1:])[reps // 2]
+        return mesh.num_elems, sorted(times[1:])[reps // 2]
 
-    med = timed(threads, 5)
-    med1 = timed(1, 2)
+    ne, med = timed(min(32, w.cfg.nx), threads, 5)
+    ne1, med1 = timed(min(4, w.cfg.nx), 1, 3)
     orc.set_threads(threads)
-    return {"value": w.mesh.num_elems / med / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
-            "value_1_thread": w.mesh.num_elems / med1 / 1e6,
-            "sample": f"64x32x32x6 Tet4 sub-box ({w.mesh.num_elems} tets), gradient_u+value, "
-                      f"median of 5 after 1 warm-up, {med:.3f} s each; 1 thread: median of 2, {med1:.3f} s"}
+    return {"value": ne / med / 1e6, "unit": "Melem/s", "cores": threads, "kind": "port",
+            "value_1_thread": ne1 / med1 / 1e6, "cpu_model": _cpu_model(), "host_cpus": os.cpu_count(),
+            "oracle_isa": isa,
+            "sample": f"x-slabs of {w.cfg.name} (same spacing and densities): gradient_u+value, "
+                      f"{min(32, w.cfg.nx)}-hex slab ({ne} tets) on {threads} threads, median of 5 after 1 "
+                      f"warm-up, {med:.3f} s each; {min(4, w.cfg.nx)}-hex slab ({ne1} tets) on 1 thread, "
+                      f"median of 3, {med1:.3f} s"}
+
+
+def density_checksum(w) -> str:
+    """First 16 hex digits of the SHA-256 of the workload's densities in node order (FP64)."""
+    import hashlib
+
+    import numpy as np
+    return hashlib.sha256(np.ascontiguousarray(w.z, np.float64).tobytes()).hexdigest()[:16]
+
+
+def cpu_threads() -> int:
+    """Threads of this process's CPU share: OMP_NUM_THREADS when set (a GPU box grants 16
+    cores per GPU and sets it), else the cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
 
 
 def main() -> int:
@@ -70,7 +115,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", default="cantilever_128x64x64")
+    ap.add_argument("--config", default="cantilever_256x128x128")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-general", action="store_true", help="skip timing the general chunk kernel")
@@ -247,15 +292,15 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        cpu = cpu_baseline(w, args.cpu_threads or cpu_threads())
 
     out = {
         "metric": METRIC, "value": value, "unit": "Melem/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": args.config, "elements": E, "nodes": N, "dofs": 3 * N, "nnz_blocks": nnzb,
-                   "step": "gradient_u + fused residual (R = K(z)u - f)", "parallelism": "single GPU"},
+                   "step": "gradient_u + fused residual (R = K(z)u - f)", "parallelism": "single GPU",
+                   "density_seed": w.cfg.seed, "density_sha256_16": density_checksum(w)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel_name, "kernel_ms": kern_ms,

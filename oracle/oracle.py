@@ -12,6 +12,32 @@ import numpy as np
 
 LIB = Path(__file__).resolve().parent / "_build" / "liboracle.so"
 _lib = None
+# (instruction-set level, /proc/cpuinfo flags it needs), best first
+_ISA_FLAGS = (("x86-64-v4", {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}),
+              ("x86-64-v3", {"avx2", "fma", "bmi2", "movbe"}))
+
+
+def select_isa() -> str:
+    """Before the first call: load the oracle build for the best instruction-set level
+    this host supports (for CPU timing); returns its name.  The default build is
+    x86-64-v2."""
+    global LIB
+    if _lib is not None:
+        raise RuntimeError("oracle already loaded")
+    try:
+        flags = set()
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                flags = set(line.split(":", 1)[1].split())
+                break
+    except OSError:
+        flags = set()
+    for isa, need in _ISA_FLAGS:
+        cand = LIB.parent / f"liboracle_{isa}.so"
+        if need <= flags and cand.exists():
+            LIB = cand
+            return isa
+    return "x86-64-v2"
 
 
 def lib():

@@ -66,19 +66,29 @@ def smoothed_density(nx: int, ny: int, nz: int, mesh: TetMesh, seed: int = 20260
     """Seeded nodal densities of the scale workloads: uniform noise on the node lattice,
     Gaussian filter (sigma in cells, reflect), standardised, sigmoid(2x) into [rho_min, 1].
     Returned in the mesh's node numbering."""
+    lat = density_lattice(nx, ny, nz, seed, sigma, rho_min)
+    c = mesh.coords
+    return lattice_to_nodes(lat, c, c[:, 0].max() / nx, c[:, 1].max() / ny, c[:, 2].max() / nz)
+
+
+def density_lattice(nx: int, ny: int, nz: int, seed: int = 20260101, sigma: float = 2.0,
+                    rho_min: float = 1e-3) -> np.ndarray:
+    """The density field of smoothed_density on the node lattice, indexed [k, j, i]."""
     from scipy.ndimage import gaussian_filter
 
     rng = np.random.default_rng(seed)
     g = rng.uniform(0.0, 1.0, size=(nz + 1, ny + 1, nx + 1))
     g = gaussian_filter(g, sigma=sigma, mode="reflect")
     g = (g - g.mean()) / g.std()
-    lat = rho_min + (1.0 - rho_min) / (1.0 + np.exp(-2.0 * g))
-    # map mesh nodes back to lattice indices
-    c = mesh.coords
-    lx, ly, lz = c[:, 0].max(), c[:, 1].max(), c[:, 2].max()
-    i = np.rint(c[:, 0] / lx * nx).astype(np.int64)
-    j = np.rint(c[:, 1] / ly * ny).astype(np.int64)
-    k = np.rint(c[:, 2] / lz * nz).astype(np.int64)
+    return rho_min + (1.0 - rho_min) / (1.0 + np.exp(-2.0 * g))
+
+
+def lattice_to_nodes(lat: np.ndarray, coords: np.ndarray, hx: float, hy: float, hz: float) -> np.ndarray:
+    """Values of a [k, j, i] lattice field at mesh nodes lying on the lattice of spacing
+    (hx, hy, hz) from the origin, in the mesh's node numbering."""
+    i = np.rint(coords[:, 0] / hx).astype(np.int64)
+    j = np.rint(coords[:, 1] / hy).astype(np.int64)
+    k = np.rint(coords[:, 2] / hz).astype(np.int64)
     return np.ascontiguousarray(lat[k, j, i])
 
 
