@@ -149,6 +149,13 @@ class ElasticModel:
         return self._h
 
     @property
+    def box_assembly(self) -> bool:
+        """True when the handle assembles the Jacobian on the lattice path (pa_box_assembly)."""
+        v = C.c_int(0)
+        _capi.call("pa_box_assembly", self._h, C.byref(v))
+        return bool(v.value)
+
+    @property
     def num_nodes(self) -> int:
         return self.mesh.num_nodes
 

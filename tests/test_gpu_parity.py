@@ -259,12 +259,18 @@ def test_gpu_errors_are_raised():
 
 # ------------------------------------------------------------------ scale
 @pytest.mark.slow
-def test_gpu_scale_cfg1_parity():
+@pytest.mark.parametrize("assembly", ["lattice", "general"])
+def test_gpu_scale_cfg1_parity(assembly, monkeypatch):
     """128x64x64 x 6 Tet4 (3.15M tets): graph bit-exact, residual / Jacobian / energy
-    gradient against the oracle on the same smoothed densities."""
-    print("case: cfg1 scale parity", flush=True)
+    gradient against the oracle on the same smoothed densities, with the lattice assembly
+    the box declaration enables and with the general chunk kernel (PA_BOX_ASSEMBLY=0, the
+    path of every unstructured or file mesh)."""
+    print(f"case: cfg1 scale parity, {assembly} assembly", flush=True)
     w = wl.make("cantilever_128x64x64")
+    if assembly == "general":
+        monkeypatch.setenv("PA_BOX_ASSEMBLY", "0")
     m = wl.model(w)
+    assert m.box_assembly == (assembly == "lattice")
     rm, ci = orc.graph(w.mesh.conn, w.mesh.num_nodes)
     np.testing.assert_array_equal(host(m.row_map), rm)
     np.testing.assert_array_equal(host(m.column_indices), ci)
