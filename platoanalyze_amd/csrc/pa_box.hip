@@ -2,15 +2,18 @@
 #include "pa_box.hpp"
 #include "pa_dpp.hpp"
 #include "pa_reduce.hpp"
+#include "pa_solver.hpp"
 
 namespace pa {
 
-__global__ __launch_bounds__(256) void k_box_material(const double* P, double* B, double lam, double mu) {
+__global__ __launch_bounds__(256) void k_box_material(const double* P, double* B, float* Bf, double lam,
+                                                      double mu) {
   for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) {
     const int ij = q % 9, i = ij / 3, j = ij % 3;
     const double* p = P + (q - ij);
     const double tr = p[0] + p[4] + p[8];
     B[q] = lam * p[ij] + mu * (p[3 * j + i] + (i == j ? tr : 0.0));
+    Bf[q] = (float)B[q];
   }
 }
 
@@ -28,8 +31,8 @@ __global__ __launch_bounds__(256) void k_box_scatter(BoxArgs A) {
   }
 }
 
-template <bool SIMP_INT>
-__global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
+template <bool SIMP_INT, typename T>
+__global__ __launch_bounds__(256) void k_box_w(BoxArgs A, T* W) {
   // One wave per run of 64 padded hexes along x: the run's line (J, K) and start are
   // wave-uniform, so the index arithmetic is 32-bit scalar work (6 nh < 2^32 was checked
   // at set-up).
@@ -45,11 +48,11 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
     const unsigned J = line % py, K = line / py;
     if (I >= px) continue;
     const unsigned hp = I + px * line;
-    typedef double v2d __attribute__((ext_vector_type(2)));
-    v2d* w2 = reinterpret_cast<v2d*>(A.W + 6 * (size_t)hp);  // 48-byte records: three 16-byte stores
+    typedef T v2 __attribute__((ext_vector_type(2)));
+    v2* w2 = reinterpret_cast<v2*>(W + 6 * (size_t)hp);  // 6-weight records: three 2-vector stores
     if (I < 1 || J < 1 || K < 1 || I > (unsigned)A.nx || J > (unsigned)A.ny || K > (unsigned)A.nz) {  // ghost hex
 #pragma unroll
-      for (int k = 0; k < 3; ++k) w2[k] = v2d{0.0, 0.0};
+      for (int k = 0; k < 3; ++k) w2[k] = v2{T(0), T(0)};
       continue;
     }
     double zc[8];  // corners of the hex whose lower corner is lattice point (I-1, J-1, K-1)
@@ -64,11 +67,13 @@ __global__ __launch_bounds__(256) void k_box_w(BoxArgs A) {
       w6[t] = SIMP_INT ? A.m.rmin + (1.0 - A.m.rmin) * simp_pow(rho, A.m.p, A.m.pint) : penalty_w(A.m, rho);
     }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) w2[k] = v2d{w6[2 * k], w6[2 * k + 1]};
+    for (int k = 0; k < 3; ++k) w2[k] = v2{(T)w6[2 * k], (T)w6[2 * k + 1]};
   }
 }
-template __global__ void k_box_w<true>(BoxArgs);
-template __global__ void k_box_w<false>(BoxArgs);
+template __global__ void k_box_w<true, double>(BoxArgs, double*);
+template __global__ void k_box_w<false, double>(BoxArgs, double*);
+template __global__ void k_box_w<true, float>(BoxArgs, float*);
+template __global__ void k_box_w<false, float>(BoxArgs, float*);
 
 // 8 lanes per graph row; lane j computes the blocks of stencil neighbours j and j + 8
 // (stencil lane 15 is the diagonal), so all rows read the same block table entries at
@@ -252,5 +257,413 @@ __global__ __launch_bounds__(256, 4) void k_box_jac(BoxArgs A) {
 
 template __global__ void k_box_jac<false>(BoxArgs);
 template __global__ void k_box_jac<true>(BoxArgs);
+
+template <bool WITH_R>
+__global__ __launch_bounds__(512, 4) void k_box_jac_ws(BoxArgs A) {
+  // LDS: two tile buffers of CRS values, two of residual rows, the block table, and the
+  // CRS range [s_base, s_end) and first row of the tile in each buffer.
+  extern __shared__ double dyn[];
+  __shared__ double sB[16 * kBoxMaxTerms * 9];
+  __shared__ int hdr[2][3];
+  const int cap = A.seg_cap;
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 512) sB[q] = A.B[q];
+  const bool producer = threadIdx.x < 256;
+  const int tid = threadIdx.x & 255;
+  const int j = tid % 8, grp = tid / 8;
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  unsigned offa[kBoxMaxTerms], offb[kBoxMaxTerms];
+#pragma unroll
+  for (int e = 0; e < kBoxMaxTerms; ++e) {
+    offa[e] = (unsigned)A.S->off[la][e];
+    offb[e] = (unsigned)A.S->off[lb][e];
+  }
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned uoffa = (unsigned)A.S->uoff[la], uoffb = (unsigned)A.S->uoff[lb];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ ul = A.ulat;
+  const double* Ba = sB + la * (kBoxMaxTerms * 9);
+  const double* Bb = sB + lb * (kBoxMaxTerms * 9);
+  const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
+  const int per = (nblk + kNumXcd - 1) / kNumXcd;
+  const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
+  const int t0 = x * per + (int)blockIdx.x / kNumXcd, t_end = min((x + 1) * per, nblk);
+  const int ntile = t0 < t_end ? (t_end - t0 + nbx - 1) / nbx : 0;
+  // row data of the producer's next tile, loaded one tile ahead (the gathers of a tile
+  // then wait on one dependent load only)
+  int n_sb = 0, n_se = 0, n_s0 = 0, n_gr = 0, n_kl = -1;
+  unsigned long long n_pm = 0;
+  auto load_rows = [&](int kk) {
+    const int base = (t0 + kk * nbx) * kBoxRows;
+    const int row = base + grp;
+    const int last = min(base + kBoxRows, A.nrows);
+    n_sb = A.rowmap[base];
+    n_se = A.rowmap[last];
+    if (row < last) {
+      n_s0 = A.rowmap[row];
+      n_gr = A.gpack[row];
+      n_pm = A.perm[row];
+      if constexpr (WITH_R) n_kl = A.lidx[row];
+    }
+  };
+  if (producer && ntile > 0) load_rows(0);
+  __syncthreads();  // block table in LDS
+  // iteration k: the producers compute tile k into buffer k & 1 while the consumers
+  // stream tile k - 1 out of the other buffer; one barrier per iteration hands over
+  for (int k = 0; k <= ntile; ++k) {
+    const int b = k & 1;
+    if (producer) {
+      if (k < ntile) {
+        double* seg = dyn + (size_t)b * 9 * cap;
+        double* rseg = dyn + (size_t)2 * 9 * cap + b * 3 * kBoxRows;
+        const int base = (t0 + k * nbx) * kBoxRows;
+        const int row = base + grp;
+        const int last = min(base + kBoxRows, A.nrows);
+        const bool live = row < last;
+        const int s_base = n_sb, s0 = n_s0, gr = live ? n_gr : 0, kl = n_kl;
+        const unsigned long long pm = n_pm;
+        if (tid == 0) {
+          hdr[b][0] = s_base;
+          hdr[b][1] = n_se;
+          hdr[b][2] = base;
+        }
+        if (k + 1 < ntile) load_rows(k + 1);
+        const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
+        const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                         (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+        const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                         (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
+        const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
+        // in-bounds gathers for every lane, as in k_box_jac
+        double wa[kBoxMaxTerms], wb[kBoxMaxTerms];
+#pragma unroll
+        for (int e = 0; e < kBoxMaxTerms; ++e) {
+          wa[e] = W[6 * hr + offa[e]];
+          wb[e] = W[6 * hr + offb[e]];
+        }
+        double dua[3] = {0.0, 0.0, 0.0}, dub[3] = {0.0, 0.0, 0.0};
+        if constexpr (WITH_R) {
+          const unsigned hca = nba ? hr + uoffa : hr, hcb = nbb ? hr + uoffb : hr;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double ur = ul[3 * hr + i];
+            dua[i] = ul[3 * hca + i] - ur;
+            dub[i] = ul[3 * hcb + i] - ur;
+          }
+        }
+        double Ka[9], Kb[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          Ka[q] = wa[0] * Ba[q];
+          Kb[q] = wb[0] * Bb[q];
+        }
+#pragma unroll
+        for (int e = 1; e < kBoxMaxTerms; ++e)
+#pragma unroll
+          for (int q = 0; q < 9; ++q) {
+            Ka[q] = fma(wa[e], Ba[9 * e + q], Ka[q]);
+            Kb[q] = fma(wb[e], Bb[9 * e + q], Kb[q]);
+          }
+        double v12[12];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v12[q] = Ka[q] + Kb[q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          v12[9 + i] = Ka[3 * i] * dua[0] + Ka[3 * i + 1] * dua[1] + Ka[3 * i + 2] * dua[2] +
+                       Kb[3 * i] * dub[0] + Kb[3 * i + 1] * dub[1] + Kb[3 * i + 2] * dub[2];
+        double sum0, sum1;
+        reduce_scatter12x8(v12, j, sum0, sum1);
+        if (live) {
+          const int i0 = rs_first(j);
+          double* dg = seg + 9 * (s0 - s_base + (int)((pm >> (4 * kBoxDiagLane)) & 15));
+          if (i0 < 9) dg[i0] = -sum0;
+          if ((j & 1) == 0 && i0 + 1 < 9) dg[i0 + 1] = -sum1;
+          if constexpr (WITH_R) {
+            if (i0 >= 9) rseg[3 * grp + i0 - 9] = sum0 - (kl >= 0 ? A.load_f[3 * (size_t)kl + (i0 - 9)] : 0.0);
+            if ((j & 1) == 0 && i0 + 1 >= 9)
+              rseg[3 * grp + i0 - 8] = sum1 - (kl >= 0 ? A.load_f[3 * (size_t)kl + (i0 - 8)] : 0.0);
+          }
+        }
+        if (nba) {
+          double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * la)) & 15));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) d[q] = Ka[q];
+        }
+        if (nbb) {
+          double* d = seg + 9 * (s0 - s_base + (int)((pm >> (4 * lb)) & 15));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) d[q] = Kb[q];
+        }
+      }
+    } else if (k > 0) {
+      // stream tile k - 1: CRS values with 16-byte non-temporal stores from the first
+      // 16-byte-aligned value on, then its residual rows (base is a multiple of 32, so
+      // 3 base doubles start on a 16-byte boundary)
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      const double* seg = dyn + (size_t)(b ^ 1) * 9 * cap;
+      const int sb = hdr[b ^ 1][0], se = hdr[b ^ 1][1], base = hdr[b ^ 1][2];
+      double* out = A.vals + 9 * (size_t)sb;
+      const int n = 9 * (se - sb);
+      const int head = sb & 1;
+      if (head && tid == 0 && n > 0) __builtin_nontemporal_store(seg[0], out);
+      for (int q = head + 2 * tid; q + 1 < n; q += 512)
+        __builtin_nontemporal_store(v2d{seg[q], seg[q + 1]}, reinterpret_cast<v2d*>(out + q));
+      if (n > head && ((n - head) & 1) && tid == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
+      if constexpr (WITH_R) {
+        const double* rseg = dyn + (size_t)2 * 9 * cap + (b ^ 1) * 3 * kBoxRows;
+        const int nr = 3 * (min(base + kBoxRows, A.nrows) - base);
+        double* rout = A.R + 3 * (size_t)base;
+        if (2 * tid + 1 < nr)
+          __builtin_nontemporal_store(v2d{rseg[2 * tid], rseg[2 * tid + 1]}, reinterpret_cast<v2d*>(rout + 2 * tid));
+        else if (2 * tid < nr)
+          __builtin_nontemporal_store(rseg[2 * tid], rout + 2 * tid);
+      }
+    }
+    lds_barrier();
+  }
+}
+template __global__ void k_box_jac_ws<false>(BoxArgs);
+template __global__ void k_box_jac_ws<true>(BoxArgs);
+
+__global__ __launch_bounds__(256, 5) void k_box_res(BoxArgs A, int sub_load) {
+  __shared__ double sB[16 * kBoxMaxTerms * 9];
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) sB[q] = A.B[q];
+  const int j = threadIdx.x % 8, grp = threadIdx.x / 8;
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  unsigned offa[kBoxMaxTerms], offb[kBoxMaxTerms];
+#pragma unroll
+  for (int e = 0; e < kBoxMaxTerms; ++e) {
+    offa[e] = (unsigned)A.S->off[la][e];
+    offb[e] = (unsigned)A.S->off[lb][e];
+  }
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned uoffa = (unsigned)A.S->uoff[la], uoffb = (unsigned)A.S->uoff[lb];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ ul = A.ulat;
+  __syncthreads();  // block table in LDS
+  const double* Ba = sB + la * (kBoxMaxTerms * 9);
+  const double* Bb = sB + lb * (kBoxMaxTerms * 9);
+  const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
+  const int per = (nblk + kNumXcd - 1) / kNumXcd;
+  const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
+  const int t_end = min((x + 1) * per, nblk);
+#pragma nounroll
+  for (int blk = x * per + (int)blockIdx.x / kNumXcd; blk < t_end; blk += nbx) {
+    // the block table is read from LDS per row, not held in registers across tiles
+    asm volatile("" ::: "memory");
+    const int row = blk * kBoxRows + grp;
+    const bool live = row < A.nrows;
+    const int gr = live ? A.gpack[row] : 0;
+    const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
+    const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+    const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
+    const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
+    double wa[kBoxMaxTerms], wb[kBoxMaxTerms];
+#pragma unroll
+    for (int e = 0; e < kBoxMaxTerms; ++e) {
+      wa[e] = W[6 * hr + offa[e]];
+      wb[e] = W[6 * hr + offb[e]];
+    }
+    const unsigned hca = nba ? hr + uoffa : hr, hcb = nbb ? hr + uoffb : hr;
+    double dua[3], dub[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double ur = ul[3 * hr + i];
+      dua[i] = ul[3 * hca + i] - ur;
+      dub[i] = ul[3 * hcb + i] - ur;
+    }
+    double Ka[9], Kb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      Ka[q] = wa[0] * Ba[q];
+      Kb[q] = wb[0] * Bb[q];
+    }
+#pragma unroll
+    for (int e = 1; e < kBoxMaxTerms; ++e)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        Ka[q] = fma(wa[e], Ba[9 * e + q], Ka[q]);
+        Kb[q] = fma(wb[e], Bb[9 * e + q], Kb[q]);
+      }
+    double s[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      s[i] = Ka[3 * i] * dua[0] + Ka[3 * i + 1] * dua[1] + Ka[3 * i + 2] * dua[2] + Kb[3 * i] * dub[0] +
+             Kb[3 * i + 1] * dub[1] + Kb[3 * i + 2] * dub[2];
+      s[i] += dpp_f64<kDppXor1>(s[i]);
+      s[i] += dpp_f64<kDppXor2>(s[i]);
+      s[i] += dpp_f64<kDppHalfMirror>(s[i]);
+    }
+    if (live && j < 3) {
+      const int k = sub_load ? A.lidx[row] : -1;
+      const double f = k >= 0 ? A.load_f[3 * (size_t)k + j] : 0.0;
+      A.R[3 * (size_t)row + j] = (j == 0 ? s[0] : j == 1 ? s[1] : s[2]) - f;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_box_diag_inv(BoxArgs A, const uint8_t* bcmask, double* dinv) {
+  __shared__ double sB[16 * kBoxMaxTerms * 9];
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) sB[q] = A.B[q];
+  __syncthreads();
+  const int j = threadIdx.x % 8;
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const double* Ba = sB + la * (kBoxMaxTerms * 9);
+  const double* Bb = sB + lb * (kBoxMaxTerms * 9);
+  const int nrow_groups = (A.nrows + 31) / 32;
+  for (int g = blockIdx.x; g < nrow_groups; g += gridDim.x) {
+    const int row = g * 32 + (int)threadIdx.x / 8;
+    const bool live = row < A.nrows;
+    const int gr = live ? A.gpack[row] : 0;
+    const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
+    const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+    const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
+    const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
+    double s[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) s[q] = 0.0;
+#pragma unroll
+    for (int e = 0; e < kBoxMaxTerms; ++e) {
+      // absent neighbours contribute nothing (their blocks are not in the row)
+      const double wa = nba ? A.W[6 * hr + (unsigned)A.S->off[la][e]] : 0.0;
+      const double wb = nbb ? A.W[6 * hr + (unsigned)A.S->off[lb][e]] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) s[q] = fma(wa, Ba[9 * e + q], fma(wb, Bb[9 * e + q], s[q]));
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      s[q] += dpp_f64<kDppXor1>(s[q]);
+      s[q] += dpp_f64<kDppXor2>(s[q]);
+      s[q] += dpp_f64<kDppHalfMirror>(s[q]);
+      s[q] = -s[q];
+    }
+    if (live && j == 0) block_inv3_masked(s, bcmask[row], dinv + 9 * (size_t)row);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_boxmf_scatter(BoxMfArgs A) {
+  const int px = A.nx + 2, py = A.ny + 2;
+  const double beta = A.first ? 0.0 : A.scal[kBeta];
+  for (int n = blockIdx.x * 256 + threadIdx.x; n < A.nrows; n += gridDim.x * 256) {
+    const int g = A.gpack[n];
+    const size_t hp = (size_t)((g & 1023) + 1) + (size_t)px * ((((g >> 10) & 1023) + 1) + (size_t)py * ((g >> 20) + 1));
+    const uint32_t bm = A.bcmask[n];
+    double v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t d = 3 * (size_t)n + i;
+      const double t = A.first ? A.z[d] : fma(beta, A.p_prev[d], A.z[d]);
+      v[i] = (bm >> i) & 1u ? 0.0 : t;
+      A.p_cur[d] = v[i];
+    }
+    A.plat[hp] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
+  }
+}
+
+// Lanes and stencil neighbours as in k_box_jac: lane j of a row takes neighbours j and
+// j + 8 (lane 15 of the stencil is the diagonal, which the differences p_c - p_r
+// absorb), so the block table reads are LDS broadcasts.  Every gather stays inside the
+// padded arrays: absent terms point at offset 0 with a zero block, ghost hexes weigh
+// zero, and a lane without a neighbour reads the row's own p.
+__global__ __launch_bounds__(256, 6) void k_boxmf_apply(BoxMfArgs A) {
+  __shared__ float sB[16 * kBoxMaxTerms * 9];
+  __shared__ double sh[4];
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 256) sB[q] = A.B[q];
+  const int j = threadIdx.x % 8, grp = threadIdx.x / 8;
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  unsigned offa[kBoxMaxTerms], offb[kBoxMaxTerms];
+#pragma unroll
+  for (int e = 0; e < kBoxMaxTerms; ++e) {
+    offa[e] = (unsigned)A.S->off[la][e];
+    offb[e] = (unsigned)A.S->off[lb][e];
+  }
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned uoffa = (unsigned)A.S->uoff[la], uoffb = (unsigned)A.S->uoff[lb];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const float* __restrict__ W = A.W;
+  const float4* __restrict__ pl = A.plat;
+  __syncthreads();  // block table in LDS
+  const float* Ba = sB + la * (kBoxMaxTerms * 9);
+  const float* Bb = sB + lb * (kBoxMaxTerms * 9);
+  const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
+  const int per = (nblk + kNumXcd - 1) / kNumXcd;
+  const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
+  const int t_end = min((x + 1) * per, nblk);
+  double dot = 0.0;
+#pragma nounroll
+  for (int blk = x * per + (int)blockIdx.x / kNumXcd; blk < t_end; blk += nbx) {
+    // the block table is read from LDS per row, not held in registers across tiles
+    asm volatile("" ::: "memory");
+    const int row = blk * kBoxRows + grp;
+    const bool live = row < A.nrows;
+    const int gr = live ? A.gpack[row] : 0;
+    const int ri = gr & 1023, rj = (gr >> 10) & 1023, rk = gr >> 20;
+    const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+    const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                     (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
+    const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
+    float wa[kBoxMaxTerms], wb[kBoxMaxTerms];
+#pragma unroll
+    for (int e = 0; e < kBoxMaxTerms; ++e) {
+      wa[e] = W[6 * hr + offa[e]];
+      wb[e] = W[6 * hr + offb[e]];
+    }
+    const float4 pr = pl[hr];
+    const float4 pa = pl[nba ? hr + uoffa : hr];
+    const float4 pb = pl[nbb ? hr + uoffb : hr];
+    const float dua[3] = {pa.x - pr.x, pa.y - pr.y, pa.z - pr.z};
+    const float dub[3] = {pb.x - pr.x, pb.y - pr.y, pb.z - pr.z};
+    float Ka[9], Kb[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      Ka[q] = wa[0] * Ba[q];
+      Kb[q] = wb[0] * Bb[q];
+    }
+#pragma unroll
+    for (int e = 1; e < kBoxMaxTerms; ++e)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        Ka[q] = fmaf(wa[e], Ba[9 * e + q], Ka[q]);
+        Kb[q] = fmaf(wb[e], Bb[9 * e + q], Kb[q]);
+      }
+    double s[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      s[i] = (double)(Ka[3 * i] * dua[0] + Ka[3 * i + 1] * dua[1] + Ka[3 * i + 2] * dua[2] +
+                      Kb[3 * i] * dub[0] + Kb[3 * i + 1] * dub[1] + Kb[3 * i + 2] * dub[2]);
+    // sum over the row's 8 lanes; every lane of the row receives the same bits
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      s[i] += dpp_f64<kDppXor1>(s[i]);
+      s[i] += dpp_f64<kDppXor2>(s[i]);
+      s[i] += dpp_f64<kDppHalfMirror>(s[i]);
+    }
+    if (live && j < 3) {
+      const size_t d = 3 * (size_t)row + j;
+      const double pc = A.p_cur[d];
+      const double yv = (A.bcmask[row] >> j) & 1u ? pc : (j == 0 ? s[0] : j == 1 ? s[1] : s[2]);
+      A.y[d] = yv;
+      dot += pc * yv;
+    }
+  }
+  const double t = block_reduce_all<256>(dot, sh);
+  if (threadIdx.x == 0) A.partials[blockIdx.x] = t;
+}
 
 }  // namespace pa

@@ -52,6 +52,7 @@ struct BoxArgs {
   double* R;
   int nrows;
   int nx, ny, nz;          // hexes per axis
+  int seg_cap;             // k_box_jac_ws: blocks of one staged tile (kBoxRows x longest row)
   Material m;
 };
 
@@ -59,17 +60,66 @@ struct BoxArgs {
 // per node: coalesced reads, scattered writes).
 __global__ void k_box_scatter(BoxArgs A);
 
-// W for every padded hex from the lattice-ordered z (one thread per hex; ghost hexes get
-// zero weights).
+// Element weights of every padded hex from the lattice-ordered z into W (T = double: the
+// assembly's weights; T = float: the FP32 matrix-free operator's); ghost hexes get zero
+// weights.  One wave per run of 64 padded hexes along x.
 // SIMP_INT: the material's penalty is SIMP with an integer exponent (Material::pint > 0).
-template <bool SIMP_INT>
-__global__ void k_box_w(BoxArgs A);
+template <bool SIMP_INT, typename T>
+__global__ void k_box_w(BoxArgs A, T* W);
 
-// B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T.
-__global__ void k_box_material(const double* P, double* B, double lam, double mu);
+// B[l][e] = lam P + mu (P^T + tr(P) I) from the unit-weight blocks P[l][e] = vol g_a g_b^T,
+// and its FP32 copy Bf.
+__global__ void k_box_material(const double* P, double* B, float* Bf, double lam, double mu);
 
 // K values of every graph row (and R = K u - f when WITH_R); 32-row tiles per work-group.
 template <bool WITH_R>
 __global__ void k_box_jac(BoxArgs A);
+
+// The same values with the work-group's waves specialised: four waves compute tile k+1
+// into one LDS buffer while four others stream tile k out of the other, so the waves
+// that wait on gathers never hold outstanding stores and the waves that store never
+// wait on loads.  512 threads; dynamic LDS of box_jac_ws_lds(seg_cap) bytes.
+template <bool WITH_R>
+__global__ void k_box_jac_ws(BoxArgs A);
+inline size_t box_jac_ws_lds(int seg_cap) { return (size_t)(2 * 9 * seg_cap + 2 * 3 * kBoxRows) * sizeof(double); }
+
+// FP32 matrix-free operator on the lattice (the mixed-precision PCG's K p and the
+// multigrid's level-0 smoother operator): y = K p with the direction update
+// p = z + beta p_prev fused, as in the chunk-based FP32 apply (constrained dofs: p = 0,
+// y = p).  K is never formed: each row sums, over its stencil neighbours c, the blocks
+// K_rc = sum_e w_e B_e (FP32 weights and unit blocks) times p_c - p_r.
+struct BoxMfArgs {
+  const int* gpack;        // per node: i | j << 10 | k << 20
+  const BoxStencil* S;
+  const float* W;          // [padded hexes][6] FP32 element weights, zero on the ghost layer
+  const float* B;          // [16][6][9] FP32 unit-weight blocks
+  const uint8_t* bcmask;   // bit i: dof 3n+i constrained
+  const double* z;         // FP64 vectors in node order
+  const double* p_prev;
+  double* p_cur;
+  int first;               // p = z (else p = z + scal[kBeta] p_prev)
+  const double* scal;
+  float4* plat;            // [padded lattice] p (FP32) at the node's lattice point
+  double* y;
+  double* partials;        // one p.y partial per work-group of k_boxmf_apply
+  int nrows;
+  int nx, ny, nz;          // hexes per axis
+};
+
+// R = K u - f (f subtracted when sub_load) without forming K, in FP64, from the element
+// weights A.W and u at the lattice points (A.ulat); the rows sum in a fixed order.
+__global__ void k_box_res(BoxArgs A, int sub_load);
+
+// Inverted 3x3 diagonal blocks of K(z) (the block-Jacobi preconditioner; constrained
+// dofs identity) from the element weights A.W: K_rr = -sum_{c != r} K_rc summed over
+// the row's lanes in a fixed order.
+__global__ void k_box_diag_inv(BoxArgs A, const uint8_t* bcmask, double* dinv);
+
+// p_cur (FP64, node order) and its FP32 copy at the lattice points (one thread per node).
+__global__ void k_boxmf_scatter(BoxMfArgs A);
+
+// y = K p_cur and the work-group's p.y partial; 8 lanes per row, 32-row tiles, a
+// persistent XCD-contiguous grid like k_box_jac.
+__global__ void k_boxmf_apply(BoxMfArgs A);
 
 }  // namespace pa

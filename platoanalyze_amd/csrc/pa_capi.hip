@@ -162,6 +162,18 @@ struct pa_problem {
   DevArray<pa::BoxStencil> box_S;
   double box_lam = 0.0, box_mu = 0.0;  // material of box_B (rebuilt when it changes)
   int box_grid_k = 1024, box_grid_r = 768;  // persistent grids of the lattice kernel (K, K and R)
+  // FP32 matrix-free operator on the lattice (k_boxmf_apply): FP32 unit blocks, per-solve
+  // FP32 element weights, p at the lattice points, one p.y partial per work-group
+  DevArray<float> box_Bf, box_W32;
+  DevArray<float4> box_plat;
+  DevArray<double> box_mf_part;
+  int box_grid_mf = 1024;
+  bool box_mf = true;  // PA_BOX_MF=0 keeps the chunk-based FP32 apply on lattice meshes
+  // k_box_jac_ws (store and compute waves specialised; PA_BOX_WS=0 selects k_box_jac)
+  bool box_ws = true;
+  int box_seg_cap = 0, box_grid_ws = 512;
+  bool box_res = true;  // lattice FP64 residual (k_box_res); PA_BOX_RES=0 keeps the chunk kernel
+  int box_grid_res = 1024;
   double* host_scal = nullptr;
 
   pa::Material material() const {
@@ -214,6 +226,9 @@ MgLevel::~MgLevel() {
 }
 
 static int box_fast_setup(pa_handle h);
+static pa::BoxArgs box_args(pa_handle h, const double* u, const double* z, double* vals, double* R);
+template <typename T>
+static void box_weights(pa_handle h, const pa::BoxArgs& a, T* W);
 static int assemble_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R);
 
 static int grid_for(size_t n, int per_block = 256, int cap = 4096) {
@@ -506,6 +521,22 @@ static int run_chunk(pa_handle h, int mode, const double* u, const double* z, do
   return PA_OK;
 }
 
+// R = K(z) u - f (f subtracted when sub): on box meshes that take the lattice assembly
+// the lattice FP64 residual (element weights of z, u at the lattice points, then the
+// rows in a fixed order), else the chunk kernel.
+static int residual(pa_handle h, const double* u, const double* z, double* R, bool sub) {
+  if (h->box_fast && h->box_res) {
+    const pa::BoxArgs a = box_args(h, u, z, nullptr, R);
+    box_weights(h, a, a.W);
+    const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
+    const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_res);
+    pa::k_box_res<<<G, 256, 0, h->stream>>>(a, sub ? 1 : 0);
+    PA_HIP(hipGetLastError());
+    return PA_OK;
+  }
+  return run_chunk(h, pa::kModeResidual, u, z, nullptr, R, sub);
+}
+
 static int sub_load(pa_handle h, double* R, double s) {
   if (h->nload == 0) return PA_OK;
   pa::k_add_load<<<grid_for(h->nload), 256, 0, h->stream>>>(R, h->load_nodes.p, h->load_f.p, h->nload, s);
@@ -516,7 +547,7 @@ static int sub_load(pa_handle h, double* R, double s) {
 int pa_vf_value(pa_handle h, const double* u, const double* z, double* R) {
   if (int rc = ready(h)) return rc;
   if (!u || !z || !R) return fail(PA_ERR_ARG, "pa_vf_value: null pointer");
-  return run_chunk(h, pa::kModeResidual, u, z, nullptr, R, true);
+  return residual(h, u, z, R, true);
 }
 
 int pa_vf_gradient_u(pa_handle h, const double* u, const double* z, double* vals, double* R) {
@@ -656,7 +687,7 @@ int pa_sf_value(pa_handle h, const double* u, const double* z, double* value) {
 int pa_sf_gradient_u(pa_handle h, const double* u, const double* z, double* g) {
   if (int rc = ready(h)) return rc;
   if (!u || !z || !g) return fail(PA_ERR_ARG, "pa_sf_gradient_u: null pointer");
-  return run_chunk(h, pa::kModeResidual, u, z, nullptr, g);
+  return residual(h, u, z, g, false);
 }
 
 int pa_sf_gradient_z(pa_handle h, const double* u, const double* z, double* g, double* value) {
@@ -806,7 +837,62 @@ static int mixed_setup(pa_handle h, const double* z) {
   da.bcmask = h->bcmask.p;
   da.dinv = h->m_dinv.p;
   da.m = h->material();
-  PA_HIP(pa::launch_mf_diag_inv(da, s));
+  if (h->box_fast && h->box_mf) {
+    // the lattice FP32 operator (element weights of this density in FP32) and the block
+    // diagonal from the FP64 weights, both summed in a fixed order
+    if (h->box_W32.n != h->box_W.n) {
+      PA_HIP(h->box_W32.alloc(h->box_W.n));
+      PA_HIP(h->box_plat.alloc(h->box_zlat.n));
+      PA_HIP(h->box_mf_part.alloc(pa::kRedBlocks));
+    }
+    const pa::BoxArgs a = box_args(h, nullptr, z, nullptr, nullptr);
+    box_weights(h, a, h->box_W32.p);
+    box_weights(h, a, a.W);
+    pa::k_box_diag_inv<<<grid_for(((size_t)nn + 31) / 32, 1, 16384), 256, 0, s>>>(a, h->bcmask.p, h->m_dinv.p);
+    PA_HIP(hipGetLastError());
+  } else {
+    PA_HIP(pa::launch_mf_diag_inv(da, s));
+  }
+  return PA_OK;
+}
+
+// y = K p through the FP32 matrix-free operator with the direction update fused (MfArgs):
+// on box meshes that take the lattice assembly the lattice apply (FP32 element weights
+// from mixed_setup, deterministic row sums), else the chunk apply.  *P and *np: the p.y
+// partials it wrote.
+static int mf_apply(pa_handle h, const pa::MfArgs& ma, const double** P, int* np) {
+  hipStream_t s = h->stream;
+  if (h->box_fast && h->box_mf) {
+    pa::BoxMfArgs b{};
+    b.gpack = h->box_gpack.p;
+    b.S = h->box_S.p;
+    b.W = h->box_W32.p;
+    b.B = h->box_Bf.p;
+    b.bcmask = h->bcmask.p;
+    b.z = ma.z;
+    b.p_prev = ma.p_prev;
+    b.p_cur = ma.p_cur;
+    b.first = ma.first;
+    b.scal = ma.scal;
+    b.plat = h->box_plat.p;
+    b.y = ma.y;
+    b.partials = h->box_mf_part.p;
+    b.nrows = h->nn;
+    b.nx = h->box.nx - 1;
+    b.ny = h->box.ny - 1;
+    b.nz = h->box.nz - 1;
+    const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
+    const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_mf);
+    pa::k_boxmf_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, s>>>(b);
+    pa::k_boxmf_apply<<<G, 256, 0, s>>>(b);
+    PA_HIP(hipGetLastError());
+    *P = h->box_mf_part.p;
+    *np = G;
+    return PA_OK;
+  }
+  PA_HIP(pa::launch_mf_apply(ma, s));
+  *P = h->chunk_partials.p;
+  *np = h->chunks.count;
   return PA_OK;
 }
 
@@ -865,7 +951,7 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   double* pbuf[2] = {h->m_p.p, h->m_p2.p};
   // true residual of u into m_r and its norm
   auto true_residual = [&](double* rr) -> int {
-    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, b == nullptr)) return rc;
+    if (int rc = residual(h, u, z, h->m_R.p, b == nullptr)) return rc;
     return masked_residual(h->m_R.p, rr);
   };
   double rr_ref = 0.0;
@@ -886,8 +972,10 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
     restart = false;
     ma.p_prev = pbuf[cur ^ 1];
     ma.p_cur = pbuf[cur];
-    PA_HIP(pa::launch_mf_apply(ma, s));
-    PA_HIP(pa::launch_pcgm_alpha(h->chunk_partials.p, h->chunks.count, h->scal.p, s));
+    const double* Pq = nullptr;
+    int npq = 0;
+    if (int rc = mf_apply(h, ma, &Pq, &npq)) return rc;
+    PA_HIP(pa::launch_pcgm_alpha(Pq, npq, h->scal.p, s));
     PA_HIP(pa::launch_pcgm_update(u, h->m_r.p, h->m_z.p, pbuf[cur], h->m_q.p, h->m_dinv.p, nn, h->scal.p,
                                   PS(pa::kPRZ0), PS(pa::kPRR), G, s));
     PA_HIP(pa::launch_pcgm_beta(PS(pa::kPRZ0), PS(pa::kPRR), G, h->scal.p, pa::kBetaStep, s));
@@ -1170,6 +1258,7 @@ static int box_fast_setup(pa_handle h) {
   }
   PA_HIP(h->box_lidx.upload(lidx));
   PA_HIP(h->box_B.alloc(P.size()));
+  PA_HIP(h->box_Bf.alloc(P.size()));
   h->box_lam = h->box_mu = 0.0;
   PA_HIP(h->box_gpack.upload(gpack));
   PA_HIP(h->box_perm.upload(perm));
@@ -1188,10 +1277,39 @@ static int box_fast_setup(pa_handle h) {
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&br, pa::k_box_jac<true>, 256, 0) != hipSuccess || br < 1)
         br = 3;
     }
+    int bm = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bm, pa::k_boxmf_apply, 256, 0) != hipSuccess || bm < 1) bm = 4;
+    h->box_grid_mf = std::min(pa::kRedBlocks, (bm * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd);
+    int bres = 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bres, pa::k_box_res, 256, 0) != hipSuccess || bres < 1) bres = 4;
+    h->box_grid_res = (bres * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
     h->box_grid_k = (bk * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
     h->box_grid_r = (br * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
     if (const char* e = std::getenv("PA_BOX_BLOCKS"))
       if (std::atoi(e) >= pa::kNumXcd) h->box_grid_k = h->box_grid_r = std::atoi(e) / pa::kNumXcd * pa::kNumXcd;
+  }
+  if (const char* e = std::getenv("PA_BOX_MF")) h->box_mf = std::atoi(e) != 0;
+  if (const char* e = std::getenv("PA_BOX_WS")) h->box_ws = std::atoi(e) != 0;
+  if (const char* e = std::getenv("PA_BOX_RES")) h->box_res = std::atoi(e) != 0;
+  h->box_seg_cap = pa::kBoxRows * g.max_row_len;
+  {
+    const size_t lds = pa::box_jac_ws_lds(h->box_seg_cap);
+    int dev = 0, cus = 256, bw = 2;
+    // the dynamic LDS of the specialised kernel exceeds the default 64 KiB limit
+    for (const void* f : {reinterpret_cast<const void*>(&pa::k_box_jac_ws<true>),
+                          reinterpret_cast<const void*>(&pa::k_box_jac_ws<false>)})
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+        (void)hipGetLastError();  // leave no error behind for the caller's next HIP call
+        h->box_ws = false;
+      }
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, pa::k_box_jac_ws<true>, 512, lds) != hipSuccess || bw < 1) {
+      (void)hipGetLastError();
+      bw = 1;
+    }
+    h->box_grid_ws = (bw * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
   }
   h->box_fast = true;
   return PA_OK;
@@ -1203,8 +1321,9 @@ int pa_box_assembly(pa_handle h, int* active) {
   return PA_OK;
 }
 
-// K (and R = K u - f) on the lattice: element weights, then the rows.
-static int box_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R) {
+// Arguments of the lattice kernels; the unit-weight material blocks (FP64 and FP32) are
+// rebuilt when the material changed.
+static pa::BoxArgs box_args(pa_handle h, const double* u, const double* z, double* vals, double* R) {
   pa::BoxArgs a{};
   a.rowmap = h->rowmap.p;
   a.col = h->colidx.p;
@@ -1228,23 +1347,47 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
   a.nz = h->box.nz - 1;
   a.m = h->material();
   if (a.m.c01 != h->box_lam || a.m.c33 != h->box_mu) {
-    pa::k_box_material<<<1, 256, 0, h->stream>>>(h->box_P.p, h->box_B.p, a.m.c01, a.m.c33);
+    pa::k_box_material<<<1, 256, 0, h->stream>>>(h->box_P.p, h->box_B.p, h->box_Bf.p, a.m.c01, a.m.c33);
     h->box_lam = a.m.c01;
     h->box_mu = a.m.c33;
   }
+  return a;
+}
+
+// z at the lattice points (and u, when a.ulat is set), then the element weights into W.
+extern "C++" template <typename T>
+static void box_weights(pa_handle h, const pa::BoxArgs& a, T* W) {
   // k_box_w: one wave per run of 64 padded hexes along x
   const size_t wv = (size_t)((a.nx + 2 + 63) / 64) * (a.ny + 2) * (a.nz + 2);
   pa::k_box_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, h->stream>>>(a);
   if (a.m.model == pa::kPenaltySimp && a.m.pint > 0)
-    pa::k_box_w<true><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a);
+    pa::k_box_w<true, T><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a, W);
   else
-    pa::k_box_w<false><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a);
+    pa::k_box_w<false, T><<<grid_for(64 * wv, 256, 16384), 256, 0, h->stream>>>(a, W);
+}
+
+// K (and R = K u - f) on the lattice: element weights, then the rows.
+static int box_jacobian(pa_handle h, const double* u, const double* z, double* vals, double* R) {
+  const pa::BoxArgs a = box_args(h, u, z, vals, R);
+  box_weights(h, a, a.W);
   const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
-  const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, R ? h->box_grid_r : h->box_grid_k);
-  if (R)
-    pa::k_box_jac<true><<<G, 256, 0, h->stream>>>(a);
-  else
-    pa::k_box_jac<false><<<G, 256, 0, h->stream>>>(a);
+  const int tiles8 = (nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+  if (h->box_ws) {
+    pa::BoxArgs aw = a;
+    aw.seg_cap = h->box_seg_cap;
+    const size_t lds = pa::box_jac_ws_lds(h->box_seg_cap);
+    const int G = std::min(tiles8, h->box_grid_ws);
+    if (R)
+      pa::k_box_jac_ws<true><<<G, 512, lds, h->stream>>>(aw);
+    else
+      pa::k_box_jac_ws<false><<<G, 512, lds, h->stream>>>(aw);
+  } else {
+    const int G = std::min(tiles8, R ? h->box_grid_r : h->box_grid_k);
+    if (R)
+      pa::k_box_jac<true><<<G, 256, 0, h->stream>>>(a);
+    else
+      pa::k_box_jac<false><<<G, 256, 0, h->stream>>>(a);
+  }
   PA_HIP(hipGetLastError());
   return PA_OK;
 }
@@ -1340,6 +1483,9 @@ static int mg_build(pa_handle h) {
     }
     if (int rc = pa_finalize(hc)) return rc;
     hc->stream = h->stream;
+    // the coarse box takes the lattice assembly (same values to rounding, summed in a
+    // fixed order); if it does not qualify, the chunk kernel assembles it
+    if (pa_set_box(hc, hx / 2, hy / 2, hz / 2) != PA_OK) hc->box_fast = false;
     if (int rc = mg_level_init(*L, grid)) return rc;
     PA_HIP(L->vals.alloc(9 * (size_t)hc->graph.rowmap[hc->nn]));
     prev_grid.swap(grid);
@@ -1376,7 +1522,9 @@ static int mg_apply(pa_handle h, size_t l, const double* x, double* y, const dou
       ma.y = y;
       ma.partials = h->chunk_partials.p;
       ma.m = h->material();
-      PA_HIP(pa::launch_mf_apply(ma, s));
+      const double* Pq = nullptr;
+      int npq = 0;
+      if (int rc = mf_apply(h, ma, &Pq, &npq)) return rc;
     } else {
       pa::k_spmv<<<pa::red_grid((size_t)h->nn, 16), 256, 0, s>>>(vals0, h->rowmap.p, h->colidx.p, x, nullptr, y,
                                                                   h->nn, 0);
@@ -1554,7 +1702,7 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
     MgLevel& F = *h->mg[l - 1];
     MgLevel& L = *h->mg[l];
     PA_HIP(pa::launch_mg_wrestrict(F.w.p, L.w.p, L.grid.p, F.node_of_grid.p, F.dims, L.nn, s));
-    if (int rc = run_chunk(L.h, pa::kModeJacobian, nullptr, L.w.p, L.vals.p, nullptr)) return rc;
+    if (int rc = assemble_jacobian(L.h, nullptr, L.w.p, L.vals.p, nullptr)) return rc;
     if (int rc = pa_apply_constraints(L.h, L.vals.p, L.rr.p, 0.0)) return rc;
     pa::k_diag_inv<<<grid_for(L.nn), 256, 0, s>>>(L.vals.p, L.h->diag_slot.p, L.dinv.p, L.nn);
     PA_HIP(hipGetLastError());
@@ -1633,7 +1781,7 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
   const double delta = std::min(0.5, std::max(1e-6, opts.inner_tolerance > 0 ? opts.inner_tolerance : 0.1));
   // true residual r = mask(b_eff - K x) into g_r (mixed path)
   auto true_residual = [&](double* rr) -> int {
-    if (int rc = run_chunk(h, pa::kModeResidual, x, z, nullptr, h->m_R.p, b == nullptr)) return rc;
+    if (int rc = residual(h, x, z, h->m_R.p, b == nullptr)) return rc;
     PA_HIP(pa::launch_neg_mask(h->m_R.p, b, h->bcmask.p, h->g_r.p, nn, PS(pa::kPDot), G, s));
     pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot1);
     PA_HIP(hipGetLastError());
@@ -1758,7 +1906,7 @@ int pa_smooth(pa_handle h, const double* z, double* u, int steps, double omega) 
   if (steps < 0 || !(omega > 0.0 && omega < 2.0)) return fail(PA_ERR_ARG, "pa_smooth: steps >= 0, 0 < omega < 2");
   if (int rc = mixed_setup(h, z)) return rc;
   for (int k = 0; k < steps; ++k) {
-    if (int rc = run_chunk(h, pa::kModeResidual, u, z, nullptr, h->m_R.p, false)) return rc;  // K u
+    if (int rc = residual(h, u, z, h->m_R.p, false)) return rc;  // K u
     PA_HIP(pa::launch_jacobi_step(u, h->m_R.p, h->m_dinv.p, h->bcmask.p, h->nn, omega, h->stream));
   }
   return PA_OK;

@@ -56,6 +56,28 @@ __device__ __forceinline__ double penalty_w(const Material& m, double rho) {
     default: return m.rmin + (1.0 - m.rmin) * simp_pow(rho, m.p, m.pint);
   }
 }
+// Inverse of a 3x3 block (row-major) by cofactors, with the constrained dofs (bits of bm)
+// replaced by identity rows and columns first: the block-Jacobi preconditioner of the
+// constrained system.
+__device__ __forceinline__ void block_inv3_masked(double a[9], uint32_t bm, double* __restrict__ o) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      if (((bm >> i) | (bm >> j)) & 1u) a[3 * i + j] = (i == j) ? 1.0 : 0.0;
+  const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
+  const double id = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
+  o[0] = c00 * id;
+  o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
+  o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
+  o[3] = c01 * id;
+  o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
+  o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
+  o[6] = c02 * id;
+  o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
+  o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+}
+
 __device__ __forceinline__ double penalty_dw(const Material& m, double rho) {
   switch (m.model) {
     case kPenaltyRamp: {

@@ -196,24 +196,7 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
     double a[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) a[q] = D[9 * lr + q];
-    const uint32_t bm = A.bcmask[r0 + lr];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        if (((bm >> i) | (bm >> j)) & 1u) a[3 * i + j] = (i == j) ? 1.0 : 0.0;
-    const double c00 = a[4] * a[8] - a[5] * a[7], c01 = a[5] * a[6] - a[3] * a[8], c02 = a[3] * a[7] - a[4] * a[6];
-    const double id = 1.0 / (a[0] * c00 + a[1] * c01 + a[2] * c02);
-    double* o = A.dinv + 9 * (size_t)(r0 + lr);
-    o[0] = c00 * id;
-    o[1] = (a[2] * a[7] - a[1] * a[8]) * id;
-    o[2] = (a[1] * a[5] - a[2] * a[4]) * id;
-    o[3] = c01 * id;
-    o[4] = (a[0] * a[8] - a[2] * a[6]) * id;
-    o[5] = (a[2] * a[3] - a[0] * a[5]) * id;
-    o[6] = c02 * id;
-    o[7] = (a[1] * a[6] - a[0] * a[7]) * id;
-    o[8] = (a[0] * a[4] - a[1] * a[3]) * id;
+    block_inv3_masked(a, A.bcmask[r0 + lr], A.dinv + 9 * (size_t)(r0 + lr));
   }
 }
 
