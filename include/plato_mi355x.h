@@ -180,6 +180,20 @@ int pa_apply_constraints(pa_handle h, double* values, double* rhs, double scale)
 int pa_solve(pa_handle h, const double* values, double* x, const double* b,
              const pa_solver_opts* opts, pa_solver_stats* stats);
 
+/* No counterpart in the reference.  pa_solve for callers that cannot allocate device
+ * memory during the call (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same block-Jacobi PCG, same arguments and results, with its
+ * workspace (four node vectors, the block-diagonal inverse, reduction partials and
+ * scalars) in a fixed device array of the library instead of handle memory allocated on
+ * first use.  Meshes of at most PA_NOALLOC_MAX_NODES nodes (more: PA_ERR_UNSUPPORTED);
+ * the array (21 * PA_NOALLOC_MAX_NODES doubles plus 6160, about 22 MB) is held on every
+ * GPU the library runs on until the process ends.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (other streams, other threads, other
+ * handles) race and must be serialised by the caller. */
+#define PA_NOALLOC_MAX_NODES 131072
+int pa_solve_noalloc(pa_handle h, const double* values, double* x, const double* b,
+                     const pa_solver_opts* opts, pa_solver_stats* stats);
+
 /* Problem::solution (src/elliptic/Problem.hpp:259-311): u = 0, rhs = -R(0,z) = f,
  * K = gradient_u, constraints, solve, u += du.  `values` is caller scratch of 9*nnz_b
  * doubles (may be NULL for precision 1, which never forms K).

@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -724,42 +725,51 @@ static int ensure_solver_ws(pa_handle h) {
   return PA_OK;
 }
 
-static int read_scal(pa_handle h) {
-  PA_HIP(hipMemcpyAsync(h->host_scal, h->scal.p, pa::kNumScal * sizeof(double), hipMemcpyDeviceToHost,
-                        h->stream));
+static int read_scal(pa_handle h, const double* scal) {
+  PA_HIP(hipMemcpyAsync(h->host_scal, scal, pa::kNumScal * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   PA_HIP(hipStreamSynchronize(h->stream));
   return PA_OK;
 }
+static int read_scal(pa_handle h) { return read_scal(h, h->scal.p); }
 
-int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const pa_solver_opts* o,
-             pa_solver_stats* st) {
-  if (int rc = ready(h)) return rc;
-  if (!vals || !x || !b) return fail(PA_ERR_ARG, "pa_solve: null pointer");
-  pa_solver_opts opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
+// Device workspace of the block-Jacobi PCG: four node vectors (3N doubles each), the 3x3
+// block-diagonal inverse (9N), kNumPartialSlots x kRedBlocks reduction partials and
+// kNumScal scalars.  Every entry is written before it is read.
+struct PcgWork {
+  double *r, *z, *p, *q, *dinv, *partials, *scal;
+};
+
+static int solve_opts(const pa_solver_opts* o, pa_solver_opts& opts, const char* who) {
+  opts = o ? *o : pa_solver_opts{1e-10, 100000, 16, 0, 0, 0.1};
   if (opts.check_every <= 0) opts.check_every = 16;
   if (opts.preconditioner != 0)
-    return fail(PA_ERR_ARG, "pa_solve: only block Jacobi (multigrid needs the density: pa_solution)");
-  if (int rc = ensure_solver_ws(h)) return rc;
+    return fail(PA_ERR_ARG, std::string(who) + ": only block Jacobi (multigrid needs the density: pa_solution)");
+  return PA_OK;
+}
+
+// Jacobi(3x3 block)-PCG in FP64 on the handle's graph, on the workspace W.
+static int pcg_block_jacobi(pa_handle h, const double* vals, double* x, const double* b,
+                            const pa_solver_opts& opts, pa_solver_stats* st, const PcgWork& W) {
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
   hipStream_t s = h->stream;
   const int G = pa::red_grid((size_t)nn, 256);       // node-parallel kernels
   const int Gs = pa::red_grid((size_t)nn, 16);       // row-group kernels (16 rows per group)
-  double* P = h->partials.p;
+  double* P = W.partials;
   auto PS = [&](int slot) { return P + (size_t)slot * pa::kRedBlocks; };
-  pa::k_diag_inv<<<grid_for(nn), 256, 0, s>>>(vals, h->diag_slot.p, h->w_dinv.p, nn);
+  pa::k_diag_inv<<<grid_for(nn), 256, 0, s>>>(vals, h->diag_slot.p, W.dinv, nn);
   if (opts.use_x0) {
-    pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p, nn, 1);
+    pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, W.r, nn, 1);
   } else {
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
-    PA_HIP(hipMemcpyAsync(h->w_r.p, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
+    PA_HIP(hipMemcpyAsync(W.r, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
-  pa::k_pcg_init<<<G, 256, 0, s>>>(h->w_r.p, h->w_z.p, h->w_p.p, b, h->w_dinv.p, nn, PS(pa::kPRZ0), PS(pa::kPRR),
+  pa::k_pcg_init<<<G, 256, 0, s>>>(W.r, W.z, W.p, b, W.dinv, nn, PS(pa::kPRZ0), PS(pa::kPRR),
                                     PS(pa::kPBB));
-  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
-  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPBB), G, h->scal.p, pa::kBB);
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, W.scal, pa::kRR);
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPBB), G, W.scal, pa::kBB);
   PA_HIP(hipGetLastError());
-  if (int rc = read_scal(h)) return rc;
+  if (int rc = read_scal(h, W.scal)) return rc;
   const double bb = h->host_scal[pa::kBB];
   double rel = bb > 0 ? std::sqrt(h->host_scal[pa::kRR] / bb) : 0.0;
   int it = 0;
@@ -768,27 +778,27 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
   while (!conv && it < opts.max_iters) {
     // rz of this iteration in P_RZ[cur]; the update writes the next one into P_RZ[cur ^ 1]
     const int cur = it & 1;
-    pa::k_pcg_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, h->w_p.p, h->w_q.p, nn, PS(pa::kPPQ));
-    pa::k_pcg_update<<<G, 256, 0, s>>>(x, h->w_r.p, h->w_z.p, h->w_p.p, h->w_q.p, h->w_dinv.p, nn, PS(pa::kPPQ),
+    pa::k_pcg_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, W.p, W.q, nn, PS(pa::kPPQ));
+    pa::k_pcg_update<<<G, 256, 0, s>>>(x, W.r, W.z, W.p, W.q, W.dinv, nn, PS(pa::kPPQ),
                                        Gs, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRR),
-                                       h->scal.p);
-    pa::k_pcg_pupdate<<<G, 256, 0, s>>>(h->w_p.p, h->w_z.p, nn, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRZ0 + cur), G);
+                                       W.scal);
+    pa::k_pcg_pupdate<<<G, 256, 0, s>>>(W.p, W.z, nn, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRZ0 + cur), G);
     ++it;
     if (it % opts.check_every == 0 || it == opts.max_iters) {
-      pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, h->scal.p, pa::kRR);
+      pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, W.scal, pa::kRR);
       PA_HIP(hipGetLastError());
-      if (int rc = read_scal(h)) return rc;
+      if (int rc = read_scal(h, W.scal)) return rc;
       rel = std::sqrt(h->host_scal[pa::kRR] / bb);
       if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "PCG produced a non-finite residual");
       conv = rel <= opts.tolerance;
     }
   }
   // true residual
-  pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, h->w_r.p, nn, 1);
-  pa::k_dot<<<G, 256, 0, s>>>(h->w_r.p, h->w_r.p, ndof, PS(pa::kPDot));
-  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, h->scal.p, pa::kDot0);
+  pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, x, b, W.r, nn, 1);
+  pa::k_dot<<<G, 256, 0, s>>>(W.r, W.r, ndof, PS(pa::kPDot));
+  pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPDot), G, W.scal, pa::kDot0);
   PA_HIP(hipGetLastError());
-  if (int rc = read_scal(h)) return rc;
+  if (int rc = read_scal(h, W.scal)) return rc;
   const double true_rel = bb > 0 ? std::sqrt(h->host_scal[pa::kDot0] / bb) : 0.0;
   if (st) {
     st->iterations = it;
@@ -800,6 +810,63 @@ int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const 
     return fail(PA_ERR_NOT_CONVERGED, "PCG reached max_iters=" + std::to_string(opts.max_iters) +
                                           " at relative residual " + std::to_string(rel));
   return PA_OK;
+}
+
+int pa_solve(pa_handle h, const double* vals, double* x, const double* b, const pa_solver_opts* o,
+             pa_solver_stats* st) {
+  if (int rc = ready(h)) return rc;
+  if (!vals || !x || !b) return fail(PA_ERR_ARG, "pa_solve: null pointer");
+  pa_solver_opts opts;
+  if (int rc = solve_opts(o, opts, "pa_solve")) return rc;
+  if (int rc = ensure_solver_ws(h)) return rc;
+  return pcg_block_jacobi(h, vals, x, b, opts, st,
+                          PcgWork{h->w_r.p, h->w_z.p, h->w_p.p, h->w_q.p, h->w_dinv.p, h->partials.p, h->scal.p});
+}
+
+// ------------------------------------------------------------ solve without allocation
+// Workspace of pa_solve_noalloc: a file-scope device array, one copy per GPU, sized for
+// meshes of up to PA_NOALLOC_MAX_NODES nodes.  Host code never uses the array's name as an
+// address (on the host it is a stand-in symbol, not device memory): the device address of
+// each GPU's copy comes from hipGetSymbolAddress, taken once per device with that device
+// current, and only that address is handed to kernels.
+constexpr size_t kNoAllocPartials = (size_t)pa::kNumPartialSlots * pa::kRedBlocks;
+constexpr size_t kNoAllocDoubles = 21 * (size_t)PA_NOALLOC_MAX_NODES + kNoAllocPartials + pa::kNumScal;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+static int noalloc_scratch(double** out) {
+  int dev = 0;
+  PA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(PA_ERR_UNSUPPORTED, "pa_solve_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PA_ERR_HIP, "pa_solve_noalloc: no device address for the workspace");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return PA_OK;
+}
+
+int pa_solve_noalloc(pa_handle h, const double* vals, double* x, const double* b, const pa_solver_opts* o,
+                     pa_solver_stats* st) {
+  if (int rc = ready(h)) return rc;
+  if (!vals || !x || !b) return fail(PA_ERR_ARG, "pa_solve_noalloc: null pointer");
+  if ((size_t)h->nn > (size_t)PA_NOALLOC_MAX_NODES)
+    return fail(PA_ERR_UNSUPPORTED, "pa_solve_noalloc: " + std::to_string(h->nn) + " nodes exceed its workspace of " +
+                                        std::to_string(PA_NOALLOC_MAX_NODES) + " nodes (use pa_solve)");
+  pa_solver_opts opts;
+  if (int rc = solve_opts(o, opts, "pa_solve_noalloc")) return rc;
+  double* ws = nullptr;
+  if (int rc = noalloc_scratch(&ws)) return rc;
+  const size_t nd = 3 * (size_t)h->nn;
+  PcgWork W{ws, ws + nd, ws + 2 * nd, ws + 3 * nd, ws + 4 * nd, ws + 4 * nd + 9 * (size_t)h->nn, nullptr};
+  W.scal = W.partials + kNoAllocPartials;
+  return pcg_block_jacobi(h, vals, x, b, opts, st, W);
 }
 
 // Mixed precision: FP64 iterative refinement around an FP32 matrix-free Jacobi-PCG.
