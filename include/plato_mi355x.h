@@ -206,6 +206,18 @@ int pa_solve_noalloc(pa_handle h, const double* values, double* x, const double*
 int pa_solution(pa_handle h, const double* z, double* u, double* values,
                 const pa_solver_opts* opts, pa_solver_stats* stats);
 
+/* No counterpart in the reference (its solves start from zero, src/elliptic/Problem.hpp:263).
+ * Problem::solution in FP64 (assembled K, 3x3 block-Jacobi PCG) deflated by the k <= 16
+ * device vectors W (column j at W + j*3N; e.g. the learned warm-start network's
+ * unit-traction responses, or earlier solutions of an optimisation run): deflated PCG
+ * (Saad, Yeung, Erhel, Guyomarc'h, SIAM J. Sci. Comput. 21(5), 2000) starts from the
+ * Galerkin correction in span(W) (added to the caller's u when use_x0 = 1) and keeps
+ * every search direction A-orthogonal to span(W).  Constrained dofs of W are ignored;
+ * A-dependent vectors are dropped.  precision 0 and preconditioner 0 only; k = 0 is
+ * pa_solution's FP64 block-Jacobi solve. */
+int pa_solution_deflated(pa_handle h, const double* z, double* u, double* values, const double* W, int k,
+                         const pa_solver_opts* opts, pa_solver_stats* stats);
+
 /* Declares the mesh a structured box of nx*ny*nz hexes (6 Tet4 each) whose nodes lie on
  * the (nx+1)(ny+1)(nz+1) lattice spanning their bounding box, in any order and
  * numbering.  Enables preconditioner 1: a geometric multigrid hierarchy of boxes with

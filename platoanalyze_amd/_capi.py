@@ -74,6 +74,7 @@ _SIGS = {
     "pa_solve": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "pa_solve_noalloc": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "pa_solution": (_I, [_P, _P, _P, _P, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
+    "pa_solution_deflated": (_I, [_P, _P, _P, _P, _P, _I, C.POINTER(SolverOpts), C.POINTER(SolverStats)]),
     "pa_compliance_gradient": (_I, [_P, _P, _P, _P, _PD]),
     "pa_stress_pnorm": (_I, [_P, _P, _P, _I, _D, _D, _PD, _P, _P]),
     "pa_stress_pnorm_gradient_x": (_I, [_P, _P, _P, _I, _D, _D, _P, _PD]),

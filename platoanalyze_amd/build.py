@@ -65,7 +65,8 @@ def build_native(verbose: bool = False) -> Path:
     obj.mkdir(exist_ok=True)
     LIBDIR.mkdir(exist_ok=True)
     inc = ["-I", str(ROOT / "include"), "-I", str(CSRC)]
-    hip_srcs = ["pa_kernels.hip", "pa_solver.hip", "pa_mf.hip", "pa_mg.hip", "pa_box.hip", "pa_capi.hip", "pa_unet.hip"]
+    hip_srcs = ["pa_kernels.hip", "pa_solver.hip", "pa_mf.hip", "pa_mg.hip", "pa_box.hip", "pa_capi.hip", "pa_unet.hip",
+                "pa_deflate.hip"]
     cmds = [[HIPCC, *HIP_FLAGS, *inc, "-x", "hip", "-c", str(CSRC / s), "-o", str(obj / (s + ".o"))]
             for s in hip_srcs]
     cmds.append(["g++", *HOST_FLAGS, *inc, "-c", str(CSRC / "pa_host.cpp"), "-o", str(obj / "pa_host.o")])

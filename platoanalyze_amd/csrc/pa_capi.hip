@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -16,6 +17,7 @@
 #include "../../include/plato_mi355x.h"
 #include "pa_host.hpp"
 #include "pa_box.hpp"
+#include "pa_deflate.hpp"
 #include "pa_kernels.hpp"
 #include "pa_mf.hpp"
 #include "pa_mg.hpp"
@@ -36,6 +38,30 @@ int fail(int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                          \
     if (e_ != hipSuccess) return fail(PA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// vol * max_a |grad N_a|^2 of a Tet4, rounded up to float (a bound on the magnitude of
+// the element's unit-weight stiffness entries divided by |lambda| + 2 |mu|).
+float element_grad_bound(const double* X, const int32_t* v) {
+  double x[4][3];
+  for (int a = 0; a < 4; ++a)
+    for (int d = 0; d < 3; ++d) x[a][d] = X[3 * (size_t)v[a] + d];
+  const double j11 = x[0][0] - x[3][0], j12 = x[1][0] - x[3][0], j13 = x[2][0] - x[3][0];
+  const double j21 = x[0][1] - x[3][1], j22 = x[1][1] - x[3][1], j23 = x[2][1] - x[3][1];
+  const double j31 = x[0][2] - x[3][2], j32 = x[1][2] - x[3][2], j33 = x[2][2] - x[3][2];
+  const double det = j11 * j22 * j33 + j12 * j23 * j31 + j13 * j21 * j32 - j11 * j23 * j32 - j12 * j21 * j33 -
+                     j13 * j22 * j31;
+  if (!(det != 0.0) || !std::isfinite(det)) return 0.0f;
+  const double inv = 1.0 / det;
+  double g[4][3] = {{(j22 * j33 - j23 * j32) * inv, (j13 * j32 - j12 * j33) * inv, (j12 * j23 - j13 * j22) * inv},
+                    {(j23 * j31 - j21 * j33) * inv, (j11 * j33 - j13 * j31) * inv, (j13 * j21 - j11 * j23) * inv},
+                    {(j21 * j32 - j22 * j31) * inv, (j12 * j31 - j11 * j32) * inv, (j11 * j22 - j12 * j21) * inv},
+                    {0, 0, 0}};
+  for (int d = 0; d < 3; ++d) g[3][d] = -(g[0][d] + g[1][d] + g[2][d]);
+  double gm = 0.0;
+  for (int a = 0; a < 4; ++a) gm = std::max(gm, g[a][0] * g[a][0] + g[a][1] * g[a][1] + g[a][2] * g[a][2]);
+  const double b = std::fabs(det) / 6.0 * gm * (1.0 + 1e-6);
+  return std::nextafter((float)b, std::numeric_limits<float>::infinity());
+}
 
 // LDS budget of one assembly work-group (two work-groups per CU fit in 160 KiB).
 constexpr int kMaxChunkBlocks = 896;
@@ -126,6 +152,10 @@ struct pa_problem {
   DevArray<double> partials;  // kNumPartialSlots x kRedBlocks
   DevArray<double> scal;
   DevArray<double> chunk_partials;
+  DevArray<float> chunk_gbound;  // per chunk: bound of vol |grad N|^2 (fixed-point Jacobian)
+  // deflated solve (pa_solution_deflated): deflation vectors, their images, partials,
+  // E = V^T A V, its inverse, mu
+  DevArray<double> d_V, d_AV, d_P, d_E, d_Einv, d_mu;
   // solver workspace (lazily allocated)
   DevArray<double> w_r, w_z, w_p, w_q, w_dinv, w_vals, w_adj, w_adjz, w_K;
   // shape derivative of the traction load: value positions in the transposed dR/dX
@@ -207,6 +237,7 @@ struct pa_problem {
     a.tslot = tslot.p;
     a.max_nodes = chunks.max_nodes;
     a.partials = chunk_partials.p;
+    a.gbound = chunk_gbound.p;
     a.load_offs = load_offs.p;
     a.load_row = load_row.p;
     a.load_f = load_f.p;
@@ -479,6 +510,16 @@ int pa_finalize(pa_handle h) {
   PA_HIP(h->scal.alloc(pa::kNumScal));
   PA_HIP(hipMemset(h->scal.p, 0, pa::kNumScal * sizeof(double)));
   PA_HIP(h->chunk_partials.alloc(std::max(1, h->chunks.count)));
+  {  // per chunk: a bound on vol |grad N|^2 over its elements (fixed-point Jacobian scale)
+    std::vector<float> eb(h->ne), gb(std::max(1, h->chunks.count), 0.0f);
+    pa::parallel_for(h->ne, [&](int64_t e0, int64_t e1) {
+      for (int64_t e = e0; e < e1; ++e) eb[e] = element_grad_bound(h->coords.data(), &h->conn[4 * (size_t)e]);
+    });
+    for (int c = 0; c < h->chunks.count; ++c)
+      for (int k = h->chunks.list_offs[c]; k < h->chunks.list_offs[c + 1]; ++k)
+        gb[c] = std::max(gb[c], eb[h->chunks.list[k] & 0x0FFFFFFFu]);
+    PA_HIP(h->chunk_gbound.upload(gb));
+  }
   PA_HIP(hipHostMalloc(&h->host_scal, pa::kNumScal * sizeof(double), hipHostMallocDefault));
   h->finalized = true;
   if (h->has_box)
@@ -747,9 +788,36 @@ static int solve_opts(const pa_solver_opts* o, pa_solver_opts& opts, const char*
   return PA_OK;
 }
 
-// Jacobi(3x3 block)-PCG in FP64 on the handle's graph, on the workspace W.
+// Deflation space of pcg_block_jacobi: k vectors V (zero at the constrained dofs), A V,
+// the inverse of E = V^T A V (k x k, device; zero rows and columns for dropped vectors),
+// mu (k) and the k x kRedBlocks partials of the projections.
+struct Deflation {
+  const double* V;
+  const double* AV;
+  int k;
+  const double* Einv;
+  double* mu;
+  double* P;
+};
+
+// y += sign * V E^-1 (X^T v) through the device kernels (X = V or A V)
+static int deflate_apply(pa_handle h, const Deflation& D, const double* X, const double* v, double* y, const double* Y,
+                         double sign, int G) {
+  const size_t ndof = 3 * (size_t)h->nn;
+  hipStream_t s = h->stream;
+  pa::k_multi_dot<<<G, 256, 0, s>>>(X, D.k, ndof, v, h->nn, D.P);
+  pa::k_defl_mu<<<1, 256, 0, s>>>(D.P, G, D.k, D.Einv, D.mu);
+  pa::k_multi_axpy<<<grid_for(ndof), 256, 0, s>>>(y, Y, D.k, ndof, D.mu, sign, ndof);
+  PA_HIP(hipGetLastError());
+  return PA_OK;
+}
+
+// Jacobi(3x3 block)-PCG in FP64 on the handle's graph, on the workspace W; with a
+// deflation space D, deflated PCG (the start corrected by V E^-1 V^T r, every direction
+// projected: p -= V E^-1 (A V)^T z).
 static int pcg_block_jacobi(pa_handle h, const double* vals, double* x, const double* b,
-                            const pa_solver_opts& opts, pa_solver_stats* st, const PcgWork& W) {
+                            const pa_solver_opts& opts, pa_solver_stats* st, const PcgWork& W,
+                            const Deflation* D = nullptr) {
   const int nn = h->nn;
   const size_t ndof = 3 * (size_t)nn;
   hipStream_t s = h->stream;
@@ -764,8 +832,14 @@ static int pcg_block_jacobi(pa_handle h, const double* vals, double* x, const do
     pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);
     PA_HIP(hipMemcpyAsync(W.r, b, ndof * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
+  if (D) {  // x += V E^-1 V^T r, r -= A V E^-1 V^T r (same mu)
+    if (int rc = deflate_apply(h, *D, D->V, W.r, x, D->V, 1.0, G)) return rc;
+    pa::k_multi_axpy<<<grid_for(ndof), 256, 0, s>>>(W.r, D->AV, D->k, ndof, D->mu, -1.0, ndof);
+  }
   pa::k_pcg_init<<<G, 256, 0, s>>>(W.r, W.z, W.p, b, W.dinv, nn, PS(pa::kPRZ0), PS(pa::kPRR),
                                     PS(pa::kPBB));
+  if (D)
+    if (int rc = deflate_apply(h, *D, D->AV, W.z, W.p, D->V, -1.0, G)) return rc;
   pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, W.scal, pa::kRR);
   pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPBB), G, W.scal, pa::kBB);
   PA_HIP(hipGetLastError());
@@ -783,6 +857,8 @@ static int pcg_block_jacobi(pa_handle h, const double* vals, double* x, const do
                                        Gs, PS(pa::kPRZ0 + cur), G, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRR),
                                        W.scal);
     pa::k_pcg_pupdate<<<G, 256, 0, s>>>(W.p, W.z, nn, PS(pa::kPRZ0 + (cur ^ 1)), PS(pa::kPRZ0 + cur), G);
+    if (D)
+      if (int rc = deflate_apply(h, *D, D->AV, W.z, W.p, D->V, -1.0, G)) return rc;
     ++it;
     if (it % opts.check_every == 0 || it == opts.max_iters) {
       pa::k_sum<<<1, 256, 0, s>>>(PS(pa::kPRR), G, W.scal, pa::kRR);
@@ -2009,6 +2085,112 @@ int pa_solution(pa_handle h, const double* z, double* u, double* vals, const pa_
   }
   if (mg) return solve_mg(h, z, u, rhs, vals, opts, st, false);
   return pa_solve(h, vals, u, rhs, &opts, st);
+}
+
+// Problem::solution (FP64, assembled K, block Jacobi) by deflated PCG over the k vectors W.
+int pa_solution_deflated(pa_handle h, const double* z, double* u, double* vals, const double* Wv, int k,
+                         const pa_solver_opts* o, pa_solver_stats* st) {
+  if (int rc = ready(h)) return rc;
+  if (!z || !u || !vals || (k > 0 && !Wv)) return fail(PA_ERR_ARG, "pa_solution_deflated: null pointer");
+  if (k < 0 || k > pa::kMaxDeflation)
+    return fail(PA_ERR_ARG, "pa_solution_deflated: k must be 0.." + std::to_string(pa::kMaxDeflation));
+  pa_solver_opts opts;
+  if (int rc = solve_opts(o, opts, "pa_solution_deflated")) return rc;
+  if (opts.precision != 0) return fail(PA_ERR_ARG, "pa_solution_deflated: FP64 (precision 0) only");
+  if (int rc = ensure_solver_ws(h)) return rc;
+  const int nn = h->nn;
+  const size_t ndof = 3 * (size_t)nn;
+  hipStream_t s = h->stream;
+  if (!h->w_vals.p || h->w_vals.n != ndof) PA_HIP(h->w_vals.alloc(ndof));
+  double* rhs = h->w_vals.p;
+  pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(rhs, ndof, 0.0);
+  if (int rc = sub_load(h, rhs, 1.0)) return rc;
+  if (int rc = assemble_jacobian(h, nullptr, z, vals, nullptr)) return rc;
+  if (int rc = pa_apply_constraints(h, vals, rhs, 1.0)) return rc;
+  const int nbc = (int)h->bc_dofs.size();
+  if (opts.use_x0 && nbc) pa::k_bc_set<<<grid_for(nbc), 256, 0, s>>>(u, h->bc_d.p, h->bc_v.p, nbc, 1.0);
+  PcgWork work{h->w_r.p, h->w_z.p, h->w_p.p, h->w_q.p, h->w_dinv.p, h->partials.p, h->scal.p};
+  if (k == 0) return pcg_block_jacobi(h, vals, u, rhs, opts, st, work);
+  // V = W with the constrained dofs zeroed, A V, E = V^T A V
+  const size_t kn = (size_t)k * ndof;
+  if (h->d_V.n < kn) {
+    PA_HIP(h->d_V.alloc(kn));
+    PA_HIP(h->d_AV.alloc(kn));
+  }
+  if (h->d_P.n == 0) {
+    PA_HIP(h->d_P.alloc((size_t)pa::kMaxDeflation * pa::kRedBlocks));
+    PA_HIP(h->d_E.alloc((size_t)pa::kMaxDeflation * pa::kMaxDeflation));
+    PA_HIP(h->d_Einv.alloc((size_t)pa::kMaxDeflation * pa::kMaxDeflation));
+    PA_HIP(h->d_mu.alloc(pa::kMaxDeflation));
+  }
+  PA_HIP(hipMemcpyAsync(h->d_V.p, Wv, kn * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const int G = pa::red_grid((size_t)nn, 256);
+  const int Gs = pa::red_grid((size_t)nn, 16);
+  std::vector<double> I((size_t)k * k, 0.0);
+  for (int j = 0; j < k; ++j) I[(size_t)j * k + j] = 1.0;
+  PA_HIP(hipMemcpyAsync(h->d_Einv.p, I.data(), I.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  for (int j = 0; j < k; ++j) {
+    double* v = h->d_V.p + (size_t)j * ndof;
+    if (nbc) pa::k_bc_zero_vec<<<grid_for(nbc), 256, 0, s>>>(v, h->bc_d.p, nbc);
+    pa::k_spmv<<<Gs, 256, 0, s>>>(vals, h->rowmap.p, h->colidx.p, v, nullptr, h->d_AV.p + (size_t)j * ndof, nn, 0);
+  }
+  for (int i = 0; i < k; ++i) {  // row i of E: (A V)^T v_i through the identity "inverse"
+    pa::k_multi_dot<<<G, 256, 0, s>>>(h->d_AV.p, k, ndof, h->d_V.p + (size_t)i * ndof, nn, h->d_P.p);
+    pa::k_defl_mu<<<1, 256, 0, s>>>(h->d_P.p, G, k, h->d_Einv.p, h->d_E.p + (size_t)i * k);
+  }
+  PA_HIP(hipGetLastError());
+  std::vector<double> E((size_t)k * k);
+  PA_HIP(hipMemcpyAsync(E.data(), h->d_E.p, E.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  PA_HIP(hipStreamSynchronize(s));
+  // inverse of E on the vectors that are A-independent: Cholesky with diagonal pivoting,
+  // a vector whose remaining pivot falls below 1e-12 of the largest diagonal is dropped
+  std::vector<double> Ein((size_t)k * k, 0.0);
+  {
+    std::vector<double> A(E);
+    for (int i = 0; i < k; ++i)
+      for (int j = 0; j < i; ++j) A[(size_t)i * k + j] = A[(size_t)j * k + i] = 0.5 * (E[(size_t)i * k + j] + E[(size_t)j * k + i]);
+    double dmax = 0.0;
+    for (int i = 0; i < k; ++i) dmax = std::max(dmax, A[(size_t)i * k + i]);
+    std::vector<int> keep;
+    std::vector<double> L;  // Cholesky factor of the kept principal submatrix, row-major m x m
+    for (int c = 0; c < k; ++c) {
+      // try to add column c: solve L y = A[keep, c], pivot = A_cc - y.y
+      const int m = (int)keep.size();
+      std::vector<double> y(m);
+      for (int i = 0; i < m; ++i) {
+        double t = A[(size_t)keep[i] * k + c];
+        for (int j = 0; j < i; ++j) t -= L[(size_t)i * k + j] * y[j];
+        y[i] = t / L[(size_t)i * k + i];
+      }
+      double piv = A[(size_t)c * k + c];
+      for (int i = 0; i < m; ++i) piv -= y[i] * y[i];
+      if (!(piv > 1e-12 * dmax)) continue;
+      L.resize((size_t)k * k, 0.0);
+      for (int i = 0; i < m; ++i) L[(size_t)m * k + i] = y[i];
+      L[(size_t)m * k + m] = std::sqrt(piv);
+      keep.push_back(c);
+    }
+    const int m = (int)keep.size();
+    // inverse of the kept block: (L L^T)^-1 column by column
+    for (int col = 0; col < m; ++col) {
+      std::vector<double> e(m, 0.0), y(m), xv(m);
+      e[col] = 1.0;
+      for (int i = 0; i < m; ++i) {
+        double t = e[i];
+        for (int j = 0; j < i; ++j) t -= L[(size_t)i * k + j] * y[j];
+        y[i] = t / L[(size_t)i * k + i];
+      }
+      for (int i = m - 1; i >= 0; --i) {
+        double t = y[i];
+        for (int j = i + 1; j < m; ++j) t -= L[(size_t)j * k + i] * xv[j];
+        xv[i] = t / L[(size_t)i * k + i];
+      }
+      for (int i = 0; i < m; ++i) Ein[(size_t)keep[i] * k + keep[col]] = xv[i];
+    }
+  }
+  PA_HIP(hipMemcpyAsync(h->d_Einv.p, Ein.data(), Ein.size() * sizeof(double), hipMemcpyHostToDevice, s));
+  const Deflation D{h->d_V.p, h->d_AV.p, k, h->d_Einv.p, h->d_mu.p, h->d_P.p};
+  return pcg_block_jacobi(h, vals, u, rhs, opts, st, work, &D);
 }
 
 // ---------------------------------------------------------------- stress p-norm + adjoint

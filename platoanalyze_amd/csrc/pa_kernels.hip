@@ -22,6 +22,23 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Order-independent LDS accumulation: v is rounded once to a multiple of 1/S (S a power
+// of two) and added as a 64-bit integer, so the sum does not depend on the order in
+// which lanes and waves arrive (integer addition is associative).  The slot holds the
+// integer until fixed_to_double converts it.
+__device__ __forceinline__ void lds_add_fixed(double* p, double v, double S) {
+  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)llrint(v * S));
+}
+
+// Scale of a chunk's fixed-point accumulation: the largest power of two S with
+// S * bound < 2^57, bound = gbound (|lambda| + 2 |mu|) on every element block entry
+// (weights <= 1), so up to 64 such contributions per entry cannot overflow and each is
+// resolved to 2^-57 of the bound.
+__device__ __forceinline__ double fixed_scale(const ChunkArgs& A, int c) {
+  const double B = (double)A.gbound[c] * (fabs(A.m.c01) + 2.0 * fabs(A.m.c33));
+  return B > 0.0 ? ldexp(1.0, 56 - ilogb(B)) : 1.0;
+}
+
 // ComputeGradientWorkset<3>: J[d][k] = X_k[d] - X_3[d]; grad N_k = row k of J^-1
 // (k<3), grad N_3 = -(sum); vol = |det J| * 1/6 (LinearTetCubRuleDegreeOne weight).
 __device__ __forceinline__ void tet_gradients(const double x[4][3], double g[4][3], double& vol) {
@@ -563,6 +580,7 @@ __device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk&
       }
   }
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
+  const double S = fixed_scale(A, c);
   for (int k = l0 + threadIdx.x; k < l1; k += NT) {
     const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
     const int ln[4] = {(int)(lw.x & 0xFFFFu), (int)(lw.x >> 16), (int)(lw.y & 0xFFFFu), (int)(lw.y >> 16)};
@@ -619,9 +637,16 @@ __device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk&
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) lds_add(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j]);
+        for (int j = 0; j < 3; ++j) lds_add_fixed(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j], S);
     }
   }
+}
+
+// Phase 2b (after the element barrier): the fixed-point segment back to doubles.
+template <int NT>
+__device__ __forceinline__ void jac_fixed_to_double(double* seg, int n, double S) {
+  const double inv = 1.0 / S;  // a power of two: exact
+  for (int k = threadIdx.x; k < n; k += NT) seg[k] = (double)__double_as_longlong(seg[k]) * inv;
 }
 
 // Phase 3 (after the element barrier): row groups finish the rows.
@@ -730,6 +755,8 @@ void k_jac_chunk(ChunkArgs A) {
   jac_stage<NT, WITH_R>(A, J, seg, nodes, roff, col, fl);
   __syncthreads();
   jac_elements<BITS, NT, WITH_R>(A, J, seg, nodes, roff, fl);
+  __syncthreads();
+  jac_fixed_to_double<NT>(seg, 9 * J.nb, fixed_scale(A, J.c));
   __syncthreads();
   jac_rows<NT, WITH_R>(A, J, seg, roff, col, fl);
   __syncthreads();

@@ -122,6 +122,9 @@ struct ChunkArgs {
   const int32_t* load_row;   // row index relative to the chunk's first row
   const double* load_f;      // 3 per entry
   int sub_load;
+  // per chunk: max over its elements of vol |grad N|_max^2 (rounded up), which bounds the
+  // magnitude of every element block entry by gbound (|lambda| + 2 mu) for weights <= 1
+  const float* gbound;
   int nchunks;
   int max_rows;
   int max_blocks;

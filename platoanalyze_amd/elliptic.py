@@ -340,6 +340,30 @@ class Problem:
         self.state = u
         return u
 
+    def solution_deflated(self, z: torch.Tensor, W: torch.Tensor, u0: torch.Tensor | None = None) -> torch.Tensor:
+        """Problem::solution by FP64 block-Jacobi PCG deflated by the rows of W (k x 3N,
+        k <= 16; pa_solution_deflated): the start is corrected by the Galerkin solution
+        in span(W) and the search directions stay A-orthogonal to it."""
+        zp = _dvec(z, self.m.num_nodes, "control z")
+        W = W.detach().to(device=z.device, dtype=torch.float64).contiguous()
+        if W.dim() != 2 or W.shape[1] != self.m.num_dofs:
+            raise ValueError(f"W: expected (k, {self.m.num_dofs}), got {tuple(W.shape)}")
+        if u0 is not None:
+            if u0.numel() != self.m.num_dofs:
+                raise ValueError(f"u0: expected {self.m.num_dofs} entries, got {u0.numel()}")
+            u = u0.detach().to(device=z.device, dtype=torch.float64).clone().contiguous()
+        else:
+            u = torch.empty(self.m.num_dofs, dtype=torch.float64, device=z.device)
+        opts = _capi.SolverOpts(self.opts.tolerance, self.opts.max_iters, self.opts.check_every, 0,
+                                1 if u0 is not None else 0, self.opts.inner_tolerance, 0)
+        if self._vals is None:
+            self._vals = torch.empty(9 * self.m.nnz_blocks, dtype=torch.float64, device=z.device)
+        self.m._stream()
+        _capi.call("pa_solution_deflated", self.m.handle, zp, _dptr(u), _dptr(self._vals), _dptr(W), W.shape[0],
+                   C.byref(opts), C.byref(self.stats))
+        self.state = u
+        return u
+
     def criterionValue(self, z: torch.Tensor) -> float:  # noqa: N802
         u = self.solution(z)
         return PhysicsScalarFunction(self.m).value(u, z)

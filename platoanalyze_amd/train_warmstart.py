@@ -114,6 +114,66 @@ def evaluate_transfer(net, c0, mode, config, tol=1e-8, log=print):
     return row
 
 
+def study_config3(base, n_traj: int = 20, n_iter: int = 100, epochs: int = 30, channels: int = 16,
+                  n_eval: int = 10, seed: int = 0, mode: str = "load", log=print) -> dict:
+    """The learned warm start on the mixed-precision / warm-start workload: n_traj seeded
+    SIMP OC trajectories of n_iter designs each, one FP64-accurate solve per design
+    (n_traj * n_iter solves), the last trajectory held out; the U-Net (4 levels, channels
+    .. 8 * channels) trained on the others by the library's BF16-MFMA kernels.  On n_eval
+    held-out designs, deterministic FP64 block-Jacobi PCG iteration counts to 1e-8: from
+    zero, from the network's guess, deflated by the guess (pa_solution_deflated), deflated
+    by the previous design's solution, and by both; FP64 multigrid from zero and from the
+    guess; and the mixed multigrid solve (the fast path) from zero and from the guess."""
+    from . import workloads as wl
+    from .elliptic import Problem
+    from .warmstart import WarmStart, train
+
+    t0 = time.perf_counter()
+    # n_traj - 1 training trajectories and one held out: n_traj * n_iter solves
+    net, c0, hist, held = train(base, n_traj - 1, n_iter, epochs, base_channels=channels, seed=seed, mode=mode,
+                                holdout=1, log=log)
+    t_train = time.perf_counter() - t0
+    idx = np.linspace(1, len(held) - 1, min(n_eval, len(held) - 1)).astype(int)
+    rows = []
+    for i in idx:
+        w, z, f = held[i]
+        m = wl.model(w)
+        ref = Problem(m, tolerance=1e-10, check_every=2, preconditioner="multigrid")
+        u_prev = ref.solution(held[i - 1][1]).clone()
+        u_ref = ref.solution(z).clone()
+        ws = WarmStart(net, w, m, c0, mode=mode)
+        g = ws.initial_guess(z, f)
+        row = {"design": int(i)}
+        pj = Problem(m, tolerance=1e-8, check_every=1)
+        pj.solution(z)
+        row["fp64_jacobi_zero"] = pj.stats.iterations
+        pj.solution(z, u0=g)
+        row["fp64_jacobi_net_start"] = pj.stats.iterations
+        for name, W in (("net", [g]), ("prev", [u_prev]), ("net_prev", [g, u_prev])):
+            pj.solution_deflated(z, torch.stack(W))
+            row[f"fp64_jacobi_deflated_{name}"] = pj.stats.iterations
+        pm = Problem(m, tolerance=1e-8, check_every=1, preconditioner="multigrid")
+        pm.solution(z)
+        row["fp64_multigrid_zero"] = pm.stats.iterations
+        pm.solution(z, u0=g)
+        row["fp64_multigrid_net_start"] = pm.stats.iterations
+        px = Problem(m, tolerance=1e-8, check_every=2, precision="mixed", preconditioner="multigrid")
+        px.solution(z)
+        row["mixed_multigrid_zero"] = px.stats.iterations
+        px.solution(z, u0=g)
+        row["mixed_multigrid_net_start"] = px.stats.iterations
+        row["guess_rel_err"] = float((g - u_ref).norm() / u_ref.norm())
+        rows.append(row)
+        log(json.dumps(row))
+    keys = [k for k in rows[0] if k not in ("design",)]
+    summary = {k: float(np.mean([r[k] for r in rows])) for k in keys}
+    out = {"config": base.cfg.name, "trajectories": n_traj, "designs_per_trajectory": n_iter,
+           "solves": n_traj * n_iter, "channels": [channels * 2 ** i for i in range(4)], "epochs": epochs,
+           "mode": mode, "train_seconds": t_train, "final_loss": hist[-1] if hist else None,
+           "mean": summary, "rows": rows}
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--config", default="cantilever_64x32x32")
@@ -131,7 +191,19 @@ def main(argv=None) -> int:
                     help="solver of the held-out evaluation (fp64: deterministic iteration counts)")
     ap.add_argument("--apply", default=None,
                     help="also apply the trained network, fully convolutionally, to this workload")
+    ap.add_argument("--study", action="store_true",
+                    help="the warm-start study (study_config3): deterministic FP64 iteration counts with and "
+                         "without the network, written as JSON to --out")
     a = ap.parse_args(argv)
+    if a.study:
+        from . import workloads as wl
+        tiny = tuple(int(v) for v in a.tiny.split(",")) if a.tiny else None
+        res = study_config3(wl.make(a.config, tiny=tiny), a.traj, a.iters, a.epochs, a.channels, seed=a.seed,
+                            mode=a.mode, log=lambda s: print(s, flush=True))
+        with open(a.out, "w") as fh:
+            json.dump(res, fh, indent=1)
+        print(json.dumps({"study": a.out, "mean": res["mean"]}), flush=True)
+        return 0
 
     from . import workloads as wl
     from .warmstart import save, train

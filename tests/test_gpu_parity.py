@@ -290,6 +290,38 @@ def test_gpu_scale_cfg1_parity(assembly, monkeypatch):
     close(host(sf.gradient_z(du, dz)), gz, RTOL, "energy grad_z")
 
 
+@pytest.mark.parametrize("mesh", ["delaunay", "box_general"])
+def test_gpu_general_jacobian_bit_identical_rerun(mesh, monkeypatch):
+    """The chunk kernel accumulates element blocks in LDS as fixed-point integers, so the
+    Jacobian and the residual do not depend on the order in which lanes and waves add
+    them: two assemblies are bit-identical (and match the oracle at 1e-12)."""
+    print(f"case: general Jacobian rerun, {mesh}", flush=True)
+    if mesh == "delaunay":
+        m_ = _random_delaunay(400, 3)
+        m = ElasticModel(m_, 1.0, 0.3, 3.0, 1e-3)
+        X = m_.coords
+        conn, nn = m_.conn, m_.num_nodes
+    else:
+        monkeypatch.setenv("PA_BOX_ASSEMBLY", "0")
+        w = wl.make("cantilever_128x64x64", tiny=(32, 16, 16))
+        m = wl.model(w)
+        assert not m.box_assembly
+        X, conn, nn = w.mesh.coords, w.mesh.conn, w.mesh.num_nodes
+    rng = np.random.default_rng(17)
+    u = rng.normal(size=3 * nn) * 1e-3
+    z = rng.uniform(0.05, 1.0, nn)
+    vf = VectorFunction(m)
+    out = []
+    for _ in range(2):
+        R = torch.empty(3 * nn, dtype=torch.float64, device="cuda")
+        K = vf.gradient_u(dev(u), dev(z), residual_out=R)
+        out.append((host(K.entries), host(R)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    rm, ci = orc.graph(conn, nn)
+    o = orc.Model(X, conn, 1.0, 0.3, 3.0, 1e-3)
+    close(out[0][0], o.jacobian(u, z, rm, ci), RTOL, "K")
+
+
 # ------------------------------------------------------------------ mixed precision
 def test_gpu_mixed_precision_solve_cfg0():
     """FP32 matrix-free PCG with FP64 residual replacement reaches the FP64 tolerance on
