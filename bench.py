@@ -194,7 +194,7 @@ def main() -> int:
 
     box = C.c_int(0)
     _capi.call("pa_box_assembly", h, C.byref(box))
-    kernel_name = ("k_box_scatter + k_box_w + k_box_jac<true> (lattice Jacobian+residual)" if box.value
+    kernel_name = ("k_box_scatter + k_box_w + k_box_jac_ws<true> (lattice Jacobian+residual)" if box.value
                    else "k_jac_chunk<4,true> (Jacobian+residual)")
 
     def timed(fn, steps, warmup):
