@@ -103,8 +103,8 @@ struct MgLevel {
   DevArray<int32_t> grid;          // (i, j, k) per node
   DevArray<int32_t> node_of_grid;  // node per grid point
   DevArray<double> r, x, y, d, t, rr, w, vals, dinv;  // w: nodal stiffness weight
-  DevArray<double> pw;             // power-iteration vector
-  double inv_theta = 0.0, c1 = 0.0, c2 = 0.0, lam = 0.0;
+  DevArray<double> pw;             // smoothing and spectral-estimate vector
+  double inv_theta = 0.0, c1 = 0.0, c2 = 0.0, lam = 0.0, lmax = 0.0;
   ~MgLevel();
 };
 
@@ -1835,8 +1835,98 @@ static int spd_inverse(std::vector<double>& A, std::vector<double>& U, int n) {
   return PA_OK;
 }
 
+// Chebyshev coefficients of level L for the interval [lmax / ratio, lmax].
+static void mg_set_interval(pa_handle h, MgLevel& L, double lmax) {
+  const double lmin = lmax / h->mg_ratio;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+  const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+  L.lmax = lmax;
+  L.inv_theta = 1.0 / theta;
+  L.c1 = rho1 * rho0;
+  L.c2 = 2.0 * rho1 / delta;
+}
+
+// Largest eigenvalue of the n x n symmetric tridiagonal (diagonal a, off-diagonal b) by
+// Sturm-sequence bisection.
+static double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b) {
+  const size_t n = a.size();
+  double lo = a[0], hi = a[0];
+  for (size_t i = 0; i < n; ++i) {  // Gershgorin interval
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < n ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {  // eigenvalues < x
+    int c = 0;
+    double q = 1.0;
+    for (size_t i = 0; i < n; ++i) {
+      const double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+      q = a[i] - x - (i > 0 ? bb / q : 0.0);
+      if (q == 0.0) q = -1e-300;
+      if (q < 0.0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-12 * std::max(1.0, std::fabs(hi)); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) == (int)n) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+// lambda_max(D^-1 K) of level l from below: the largest Ritz value of kLanczosSteps steps
+// of block-Jacobi PCG on K x = v (v a fixed pseudo-random vector), whose coefficients give
+// the Lanczos tridiagonal T_jj = 1/alpha_j + beta_{j-1}/alpha_{j-1},
+// T_j,j+1 = sqrt(beta_j)/alpha_j (Saad, Iterative Methods for Sparse Linear Systems, 6.7.3).
+// Power iteration from the same start converges too slowly on high-contrast SIMP designs,
+// whose top eigenvectors sit on a few solid/void interfaces (a 12-step power estimate of
+// 2.13 against a true 2.63 made the smoother divergent there).
+static int mg_lambda_max(pa_handle h, size_t l, const double* vals0, bool mixed, double* lam) {
+  constexpr int kLanczosSteps = 16;
+  hipStream_t s = h->stream;
+  MgLevel& L = *h->mg[l];
+  const size_t nd = 3 * (size_t)L.nn;
+  const double* dinv = mg_dinv(h, l, mixed);
+  double *r = L.r.p, *z = L.y.p, *p = L.pw.p, *q = L.t.p;
+  std::vector<double> v0(nd);
+  for (size_t i = 0; i < nd; ++i) v0[i] = (double)((i * 2654435761ull >> 7) % 2001) / 1000.0 - 1.0;
+  PA_HIP(hipMemcpyAsync(r, v0.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
+  PA_HIP(pa::launch_dinv_apply(z, dinv, r, L.nn, s));
+  PA_HIP(hipMemcpyAsync(p, z, nd * sizeof(double), hipMemcpyDeviceToDevice, s));
+  double rz = 0.0;
+  if (int rc = mg_dot(h, r, z, nd, pa::kDot0, &rz)) return rc;
+  std::vector<double> alpha, beta, ta, tb;
+  const double rz0 = rz;
+  for (int j = 0; j < kLanczosSteps && rz > 1e-28 * rz0; ++j) {
+    if (int rc = mg_apply(h, l, p, q, vals0, mixed)) return rc;
+    double pq = 0.0;
+    if (int rc = mg_dot(h, p, q, nd, pa::kDot0, &pq)) return rc;
+    if (!(pq > 0.0) || !std::isfinite(pq)) break;
+    const double a = rz / pq;
+    const size_t g = grid_for(nd);
+    pa::k_axpy<<<g, 256, 0, s>>>(r, q, nd, -a);
+    PA_HIP(pa::launch_dinv_apply(z, dinv, r, L.nn, s));
+    double rz1 = 0.0;
+    if (int rc = mg_dot(h, r, z, nd, pa::kDot0, &rz1)) return rc;
+    const double b = rz1 / rz;
+    ta.push_back(1.0 / a + (j > 0 ? beta.back() / alpha.back() : 0.0));
+    if (j > 0) tb.push_back(std::sqrt(beta.back()) / alpha.back());
+    alpha.push_back(a);
+    beta.push_back(b);
+    pa::k_scale<<<g, 256, 0, s>>>(p, nd, b);
+    pa::k_axpy<<<g, 256, 0, s>>>(p, z, nd, 1.0);
+    PA_HIP(hipGetLastError());
+    rz = rz1;
+  }
+  if (ta.empty()) return fail(PA_ERR_NOT_CONVERGED, "multigrid: bad spectral estimate");
+  *lam = tridiag_max_eig(ta, tb);
+  if (!(*lam > 0.0) || !std::isfinite(*lam)) return fail(PA_ERR_NOT_CONVERGED, "multigrid: bad spectral estimate");
+  return PA_OK;
+}
+
 // Per solve: homogenised coarse densities (restricted SIMP weights), coarse Jacobians,
-// their block diagonals, Chebyshev intervals by power iteration, coarsest inverse.
+// their block diagonals, Chebyshev intervals from Lanczos estimates, coarsest inverse.
 static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixed) {
   hipStream_t s = h->stream;
   const size_t nl = h->mg.size();
@@ -1850,33 +1940,15 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
     pa::k_diag_inv<<<grid_for(L.nn), 256, 0, s>>>(L.vals.p, L.h->diag_slot.p, L.dinv.p, L.nn);
     PA_HIP(hipGetLastError());
   }
-  // Chebyshev interval [lmax / 30, 1.1 lmax] of D^-1 K per smoothed level
+  // Chebyshev interval [lmax / ratio, lmax], lmax = 1.1 x the largest Ritz value of D^-1 K
+  // per smoothed level
   for (size_t l = 0; l + 1 < nl; ++l) {
     MgLevel& L = *h->mg[l];
-    const size_t nd = 3 * (size_t)L.nn;
-    const double* dinv = mg_dinv(h, l, mixed);
-    std::vector<double> v0(nd);
-    for (size_t i = 0; i < nd; ++i) v0[i] = (double)((i * 2654435761ull >> 7) % 2001) / 1000.0 - 1.0;
-    PA_HIP(hipMemcpyAsync(L.pw.p, v0.data(), nd * sizeof(double), hipMemcpyHostToDevice, s));
-    // v <- D^-1 K v / |D^-1 K v| on the device; the host reads |D^-1 K v| once
-    if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, nullptr)) return rc;
-    double nrm = 0.0;
-    for (int it = 0; it < 12; ++it) {
-      PA_HIP(pa::launch_scale_rsqrt(L.pw.p, nd, h->scal.p, pa::kDot0, s));
-      if (int rc = mg_apply(h, l, L.pw.p, L.t.p, vals0, mixed)) return rc;
-      PA_HIP(pa::launch_dinv_apply(L.pw.p, dinv, L.t.p, L.nn, s));
-      if (int rc = mg_dot(h, L.pw.p, L.pw.p, nd, pa::kDot0, it == 11 ? &nrm : nullptr)) return rc;
-    }
-    const double lam = std::sqrt(nrm);
-    if (!(lam > 0.0) || !std::isfinite(lam)) return fail(PA_ERR_NOT_CONVERGED, "multigrid: bad spectral estimate");
-    const double lmax = 1.1 * lam, lmin = lmax / h->mg_ratio;
-    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-    const double sigma = theta / delta, rho0 = 1.0 / sigma, rho1 = 1.0 / (2.0 * sigma - rho0);
+    double lam = 0.0;
+    if (int rc = mg_lambda_max(h, l, vals0, mixed, &lam)) return rc;
     L.lam = lam;
     if (mg_trace()) std::fprintf(stderr, "mg level %zu: %d nodes, lambda_max(D^-1 K) ~ %.6g\n", l, L.nn, lam);
-    L.inv_theta = 1.0 / theta;
-    L.c1 = rho1 * rho0;
-    L.c2 = 2.0 * rho1 / delta;
+    mg_set_interval(h, L, 1.1 * lam);
   }
   // coarsest level: dense inverse on the host (a few hundred dofs)
   MgLevel& Lc = *h->mg.back();
@@ -1895,6 +1967,11 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
   PA_HIP(hipStreamSynchronize(s));
   return PA_OK;
 }
+
+// A negative r.V(r) during the multigrid PCG widens every Chebyshev interval by kMgWiden
+// and restarts, at most kMgMaxWidenings times per solve.
+constexpr double kMgWiden = 1.5;
+constexpr int kMgMaxWidenings = 3;
 
 // Multigrid-preconditioned CG.  FP64 path (mixed = false): the assembled, constrained
 // K in `vals0`, right-hand side b, x from zero (or x0).  Mixed path: the FP32
@@ -1970,7 +2047,7 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
   if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
   bool conv = bb == 0.0 || rel <= opts.tolerance;
   if (bb == 0.0) pa::k_fill<<<grid_for(ndof), 256, 0, s>>>(x, ndof, 0.0);  // b = 0: the solution is 0, whatever x0
-  int it = 0, replacements = 0;
+  int it = 0, replacements = 0, widenings = 0;
   double rr_ref = rr0;
   // z = V(r) and rz; kBetaInit also starts p = z, kBetaReplace recomputes beta for a
   // replaced residual (the direction update p = z + beta p follows in the loop)
@@ -2004,7 +2081,31 @@ static int solve_mg(pa_handle h, const double* z, double* x, const double* b, co
         std::fprintf(stderr, "mg pcg it %d: iterated rel %.3e, rz %.3e, alpha %.3e, beta %.3e\n", it,
                      std::sqrt(rr_it / bb), h->host_scal[pa::kRZ], h->host_scal[pa::kAlpha], h->host_scal[pa::kBeta]);
       if (!std::isfinite(rr_it)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
-      if (!mixed) {
+      // r.V(r) <= 0: the V-cycle is indefinite (a smoother interval misses the top of its
+      // level's spectrum).  Widen every interval and restart PCG from the true residual.
+      const bool indefinite = !(h->host_scal[pa::kRZ] > 0.0) && rr_it > opts.tolerance * opts.tolerance * bb;
+      if (indefinite && !last && widenings < kMgMaxWidenings) {
+        ++widenings;
+        for (size_t l = 0; l + 1 < h->mg.size(); ++l) mg_set_interval(h, *h->mg[l], kMgWiden * h->mg[l]->lmax);
+        if (mixed) {
+          ++replacements;
+          if (int rc = true_residual(&rr_ref)) return rc;
+          rel = std::sqrt(rr_ref / bb);
+        } else {
+          pa::k_spmv<<<pa::red_grid((size_t)nn, 16), 256, 0, s>>>(vals0, h->rowmap.p, h->colidx.p, x, b, h->g_r.p,
+                                                                  nn, 1);
+          double rr = 0.0;
+          if (int rc = mg_dot(h, h->g_r.p, h->g_r.p, ndof, pa::kDot0, &rr)) return rc;
+          rel = std::sqrt(rr / bb);
+        }
+        if (mg_trace()) std::fprintf(stderr, "mg pcg it %d: indefinite V-cycle, intervals widened, true rel %.3e\n", it, rel);
+        if (!std::isfinite(rel)) return fail(PA_ERR_NOT_CONVERGED, "multigrid PCG: non-finite residual");
+        conv = rel <= opts.tolerance;
+        if (!conv) {
+          if (int rc = precondition(pa::kBetaInit)) return rc;
+          restarted = true;
+        }
+      } else if (!mixed) {
         rel = std::sqrt(rr_it / bb);
         conv = rel <= opts.tolerance;
       } else if (rr_it <= delta * delta * rr_ref || rr_it <= opts.tolerance * opts.tolerance * bb || last) {
