@@ -110,6 +110,44 @@ def cpu_threads() -> int:
     return len(os.sched_getaffinity(0))
 
 
+def profile_unet(args) -> int:
+    """The warm-start U-Net on the node grid of the workload (BF16 MFMA convolutions):
+    HIP-event time of a forward pass and of a gradient (forward + backward), and the
+    forward convolutions' dense FLOP rate against the BF16 MFMA peak."""
+    import torch
+
+    from platoanalyze_amd import workloads as wl
+    from platoanalyze_amd.unet import IN_CH, OUT_CH, UNet
+
+    c = wl.CONFIGS[args.config]
+    grid, batch = (c.nx + 1, c.ny + 1, c.nz + 1), 4
+    net = UNet(grid, args.unet_channels, seed=0)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((batch, *grid, IN_CH), generator=g, device="cuda")
+    y = torch.randn((batch, *grid, OUT_CH), generator=g, device="cuda") * 0.1
+    out = {}
+    passes = [("forward", lambda: net.forward(x))]
+    if not args.unet_forward_only:
+        passes.append(("gradient", lambda: net.gradient(x, y)))
+    for name, fn in passes:
+        for _ in range(max(args.warmup, 1)):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.profile_unet):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = {"ms": e0.elapsed_time(e1) / args.profile_unet}
+    fl = net.forward_flops(batch)
+    out["forward"].update({"flops": fl, "TFLOPs": fl / (out["forward"]["ms"] * 1e-3) / 1e12})
+    out["forward"]["frac_bf16_dense_peak"] = out["forward"]["TFLOPs"] / 2500.0
+    print(json.dumps({"profile_unet": out, "grid": grid, "batch": batch, "channels": args.unet_channels,
+                      "config": args.config}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +163,14 @@ def main() -> int:
     ap.add_argument("--profile-solve", type=int, default=0, metavar="K",
                     help="run only the solves (FP64/mixed x Jacobi/multigrid), each capped at K iterations "
                          "(for profilers)")
+    ap.add_argument("--profile-unet", type=int, default=0, metavar="K",
+                    help="run only the warm-start U-Net: K timed forward passes, then K gradients "
+                         "(node grid of --config, --unet-channels, batch 4)")
+    ap.add_argument("--unet-channels", type=int, default=16)
+    ap.add_argument("--unet-forward-only", action="store_true")
+    ap.add_argument("--warm-start-study", action="store_true",
+                    help="also run the learned warm-start study at 64x32x32 (about 4 minutes) and report it "
+                         "in place of the committed one")
     ap.add_argument("--solvers", default="fp64_jacobi,mixed_jacobi,fp64_multigrid,mixed_multigrid",
                     help="comma-separated precision_preconditioner variants for --profile-solve")
     args = ap.parse_args()
@@ -150,6 +196,8 @@ def main() -> int:
         print(json.dumps({"copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8,
                           "launches": args.warmup + args.steps}))
         return 0
+    if args.profile_unet:
+        return profile_unet(args)
     if args.profile_solve:
         w = wl.make(args.config)
         m = wl.model(w, device=dev)
@@ -289,6 +337,24 @@ def main() -> int:
                           "speedup_vs_fp64_jacobi": a["seconds"] / r["seconds"]})
             solve_out[f"{key[0]}_{key[1]}"] = d
         extra["solve"] = solve_out
+
+    # the learned warm start on the 64x32x32 cantilever: deterministic FP64 iteration
+    # counts with and without the network (20 OC trajectories, 2000 solves, U-Net 16-128
+    # channels trained by the library's kernels), live or from the committed study
+    ws_file = ROOT / "profiles" / "warm_start_cantilever_64x32x32.json"
+    if args.warm_start_study:
+        from platoanalyze_amd.train_warmstart import study_config3
+
+        st = study_config3(wl.make("cantilever_64x32x32"), 20, 100, 30, 16, log=lambda s_: None)
+        st["source"] = "measured in this run"
+    elif ws_file.exists():
+        st = json.loads(ws_file.read_text())
+        st["source"] = f"{ws_file.relative_to(ROOT)} (committed; bench.py --warm-start-study reruns it)"
+    else:
+        st = None
+    if st is not None:
+        extra["warm_start_study"] = {k: st.get(k) for k in ("config", "source", "solves", "channels", "epochs",
+                                                            "train_seconds", "final_loss", "mean", "checks")}
 
     cpu = None
     if not args.no_cpu:

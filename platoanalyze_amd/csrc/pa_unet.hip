@@ -18,6 +18,8 @@ typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kT = 256;
+constexpr int kConvFillWaves = 2048;  // eight waves per CU of a 256-CU part
+constexpr int kConvMaxSlices = 32;
 
 __device__ __forceinline__ bf16_t f2bf(float f) {  // round to nearest even
   const uint32_t u = __float_as_uint(f);
@@ -60,15 +62,31 @@ __device__ __forceinline__ void decode(const Grid& g, int o, int& x, int& y, int
   x = r / g.ny;
 }
 
-// One work-group = 4 wavefronts; a wavefront computes 16 output voxels x 64 channels.
-template <bool XF32>
-__global__ __launch_bounds__(kT) void k_conv(ConvArgs A) {
+// Bias, ReLU, gate and the stores of output element (m, n) with the reduction sum v.
+__device__ __forceinline__ void conv_epilogue(const ConvArgs& A, int m, int n, float v) {
+  if (A.bias) v += A.bias[n];
+  if (A.relu) v = fmaxf(v, 0.0f);
+  if (A.gate && !bf_pos(A.gate[(size_t)m * A.ldg + A.gcol + n])) v = 0.0f;
+  if (A.Y) A.Y[(size_t)m * A.ldy + A.ycol + n] = f2bf(v);
+  if (A.Yf) {
+    float* d = A.Yf + (size_t)m * A.ldyf + A.yfcol + n;
+    *d = A.accumulate ? *d + v : v;
+  }
+}
+
+// One work-group = 4 wavefronts; a wavefront computes 16 output voxels x 64 channels over
+// K steps [blockIdx.z * kper, (blockIdx.z + 1) * kper) of 32 (SPLIT: raw sums into the
+// slice's part array; else the whole reduction and the epilogue).
+template <bool XF32, bool SPLIT>
+__global__ __launch_bounds__(kT) void k_conv(ConvArgs A, int kper) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int Vo = A.go.count(), Vi = A.gi.count();
   const int Mtot = A.batch * Vo;
   const int m0 = (blockIdx.x * 4 + wave) * 16;
   const int n0 = blockIdx.y * 64;
   const int Ktot = 27 * A.cin;
+  const int kbeg = SPLIT ? (int)blockIdx.z * kper * 32 : 0;
+  const int kend = SPLIT ? min(Ktot, kbeg + kper * 32) : Ktot;
   const int mrow = m0 + (lane & 15);
   const bool mvalid = mrow < Mtot;
   int b = 0, ox = 0, oy = 0, oz = 0;
@@ -82,8 +100,8 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs A) {
   for (int j = 0; j < 4; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
   // this lane's k offset inside a 32-wide K step, tracked as (tap, channel)
   const int kq = 8 * (lane >> 4);
-  int t = kq / A.cin, ch = kq - (kq / A.cin) * A.cin;
-  for (int kc = 0; kc < Ktot; kc += 32) {
+  int t = (kbeg + kq) / A.cin, ch = (kbeg + kq) - ((kbeg + kq) / A.cin) * A.cin;
+  for (int kc = kbeg; kc < kend; kc += 32) {
     const int k0 = kc + kq;
     bf16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
     if (mvalid && k0 < Ktot) {
@@ -117,20 +135,23 @@ __global__ __launch_bounds__(kT) void k_conv(ConvArgs A) {
   for (int j = 0; j < 4; ++j) {
     const int n = n0 + 16 * j + (lane & 15);
     if (n >= A.cout) continue;
-    const float bias = A.bias ? A.bias[n] : 0.0f;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = m0 + 4 * (lane >> 4) + i;
       if (m >= Mtot) continue;
-      float v = acc[j][i] + bias;
-      if (A.relu) v = fmaxf(v, 0.0f);
-      if (A.gate && !bf_pos(A.gate[(size_t)m * A.ldg + A.gcol + n])) v = 0.0f;
-      if (A.Y) A.Y[(size_t)m * A.ldy + A.ycol + n] = f2bf(v);
-      if (A.Yf) {
-        float* d = A.Yf + (size_t)m * A.ldyf + A.yfcol + n;
-        *d = A.accumulate ? *d + v : v;
-      }
+      if constexpr (SPLIT) A.part[((size_t)blockIdx.z * Mtot + m) * A.cout + n] = acc[j][i];
+      else conv_epilogue(A, m, n, acc[j][i]);
     }
+  }
+}
+
+// Sum of the slices of a split reduction, in slice order, and the epilogue.
+__global__ __launch_bounds__(kT) void k_conv_reduce(ConvArgs A, int nslices) {
+  const size_t Mtot = (size_t)A.batch * A.go.count(), total = Mtot * A.cout;
+  for (size_t e = blockIdx.x * (size_t)kT + threadIdx.x; e < total; e += (size_t)gridDim.x * kT) {
+    float v = A.part[e];
+    for (int z = 1; z < nslices; ++z) v += A.part[(size_t)z * total + e];
+    conv_epilogue(A, (int)(e / A.cout), (int)(e % A.cout), v);
   }
 }
 
@@ -254,9 +275,26 @@ int blocks_for(size_t n, int cap = 4096) {
 
 hipError_t launch_conv(const ConvArgs& a, hipStream_t s) {
   const int M = a.batch * a.go.count();
+  const int waves = ((M + 15) / 16) * ((a.cout + 63) / 64);
+  const int ksteps = (27 * a.cin + 31) / 32;
+  // slices of at least two K steps until the grid holds about kConvFillWaves waves
+  int nsl = 1;
+  if (a.part && waves < kConvFillWaves) {
+    nsl = std::min({(kConvFillWaves + waves - 1) / waves, ksteps / 2, kConvMaxSlices});
+    while (nsl > 1 && (size_t)nsl * M * a.cout > a.part_cap) --nsl;
+  }
   dim3 grid((M + 63) / 64, (a.cout + 63) / 64);
-  if (a.X) k_conv<false><<<grid, kT, 0, s>>>(a);
-  else k_conv<true><<<grid, kT, 0, s>>>(a);
+  if (nsl <= 1) {
+    if (a.X) k_conv<false, false><<<grid, kT, 0, s>>>(a, ksteps);
+    else k_conv<true, false><<<grid, kT, 0, s>>>(a, ksteps);
+    return hipGetLastError();
+  }
+  const int kper = (ksteps + nsl - 1) / nsl;
+  nsl = (ksteps + kper - 1) / kper;  // every slice non-empty
+  grid.z = nsl;
+  if (a.X) k_conv<false, true><<<grid, kT, 0, s>>>(a, kper);
+  else k_conv<true, true><<<grid, kT, 0, s>>>(a, kper);
+  k_conv_reduce<<<blocks_for((size_t)M * a.cout), kT, 0, s>>>(a, nsl);
   return hipGetLastError();
 }
 
@@ -374,6 +412,7 @@ struct pa_unet_s {
   Dev<float> out, dout;
   Dev<bf16_t> in;
   Dev<double> partials;
+  Dev<float> split;  // split-reduction slices of the small convolutions
   double* host_partials = nullptr;
   int step = 0;
   hipStream_t stream = nullptr;
@@ -407,6 +446,7 @@ struct Buf {
   int level, ch;
 };
 constexpr int kNumBuf = 12;
+constexpr size_t kSplitFloats = (size_t)8 << 20;  // 32 MB of split-reduction slices
 
 void build_layers(pa_unet h) {
   const int C = h->base;
@@ -475,6 +515,7 @@ int ensure_batch(pa_unet h, int batch) {
   const size_t no = (size_t)batch * h->g[0].count() * kOutCh;
   PA_UHIP(h, h->out.alloc(no));
   PA_UHIP(h, h->dout.alloc(no));
+  if (!h->split.p) PA_UHIP(h, h->split.alloc(kSplitFloats));
   h->batch = batch;
   return PA_OK;
 }
@@ -503,6 +544,8 @@ pa::unet::ConvArgs fwd_args(pa_unet h, const Layer& l) {
     a.Yf = h->out.p;
     a.ldyf = kOutCh;
   }
+  a.part = h->split.p;
+  a.part_cap = h->split.n;
   return a;
 }
 
@@ -558,6 +601,8 @@ int backward(pa_unet h) {
     a.gate = h->act[bi].p;  // every layer input except the network input is a ReLU output
     a.ldg = l.in.ld;
     a.gcol = l.in.col;
+    a.part = h->split.p;
+    a.part_cap = h->split.n;
     PA_UHIP(h, pa::unet::launch_conv(a, h->stream));
   }
   return PA_OK;

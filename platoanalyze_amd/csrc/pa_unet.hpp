@@ -42,6 +42,12 @@ struct ConvArgs {
   int accumulate;     // Yf += result instead of =
   const bf16_t* gate; // multiply the result by (gate > 0) (ReLU derivative), same shape as Yf
   int ldg, gcol;
+  // Split reduction for grids too small to fill the GPU: work-group slice z sums its range
+  // of the 27*cin reduction into part[z][M][cout] (FP32, M = batch x output voxels) and a
+  // second kernel adds the slices in slice order and applies the epilogue.  part null:
+  // one pass.  part_cap: floats available at part.
+  float* part;
+  size_t part_cap;
 };
 hipError_t launch_conv(const ConvArgs& a, hipStream_t s);
 

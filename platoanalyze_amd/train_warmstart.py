@@ -115,7 +115,8 @@ def evaluate_transfer(net, c0, mode, config, tol=1e-8, log=print):
 
 
 def study_config3(base, n_traj: int = 20, n_iter: int = 100, epochs: int = 30, channels: int = 16,
-                  n_eval: int = 10, seed: int = 0, mode: str = "load", log=print) -> dict:
+                  n_eval: int = 10, seed: int = 0, mode: str = "load", lr: float = 3e-4, batch: int = 4,
+                  log=print) -> dict:
     """The learned warm start on the mixed-precision / warm-start workload: n_traj seeded
     SIMP OC trajectories of n_iter designs each, one FP64-accurate solve per design
     (n_traj * n_iter solves), the last trajectory held out; the U-Net (4 levels, channels
@@ -130,8 +131,8 @@ def study_config3(base, n_traj: int = 20, n_iter: int = 100, epochs: int = 30, c
 
     t0 = time.perf_counter()
     # n_traj - 1 training trajectories and one held out: n_traj * n_iter solves
-    net, c0, hist, held = train(base, n_traj - 1, n_iter, epochs, base_channels=channels, seed=seed, mode=mode,
-                                holdout=1, log=log)
+    net, c0, hist, held = train(base, n_traj - 1, n_iter, epochs, batch=batch, lr=lr, base_channels=channels,
+                                seed=seed, mode=mode, holdout=1, log=log)
     t_train = time.perf_counter() - t0
     idx = np.linspace(1, len(held) - 1, min(n_eval, len(held) - 1)).astype(int)
     rows = []
@@ -149,8 +150,9 @@ def study_config3(base, n_traj: int = 20, n_iter: int = 100, epochs: int = 30, c
         row["fp64_jacobi_zero"] = pj.stats.iterations
         pj.solution(z, u0=g)
         row["fp64_jacobi_net_start"] = pj.stats.iterations
-        for name, W in (("net", [g]), ("prev", [u_prev]), ("net_prev", [g, u_prev])):
-            pj.solution_deflated(z, torch.stack(W))
+        for name, W in (("net", ws.deflation_space(z, f)), ("prev", u_prev[None, :]),
+                        ("net_prev", ws.deflation_space(z, f, u_prev))):
+            pj.solution_deflated(z, W)
             row[f"fp64_jacobi_deflated_{name}"] = pj.stats.iterations
         pm = Problem(m, tolerance=1e-8, check_every=1, preconditioner="multigrid")
         pm.solution(z)
@@ -169,9 +171,22 @@ def study_config3(base, n_traj: int = 20, n_iter: int = 100, epochs: int = 30, c
     summary = {k: float(np.mean([r[k] for r in rows])) for k in keys}
     out = {"config": base.cfg.name, "trajectories": n_traj, "designs_per_trajectory": n_iter,
            "solves": n_traj * n_iter, "channels": [channels * 2 ** i for i in range(4)], "epochs": epochs,
-           "mode": mode, "train_seconds": t_train, "final_loss": hist[-1] if hist else None,
-           "mean": summary, "rows": rows}
+           "mode": mode, "lr": lr, "batch": batch, "train_seconds": t_train,
+           "loss_history": [float(v) for v in hist], "final_loss": hist[-1] if hist else None,
+           "mean": summary, "checks": study_checks(rows), "rows": rows}
     return out
+
+
+def study_checks(rows: list[dict]) -> dict:
+    """Per-design comparisons of the deterministic FP64 block-Jacobi counts: does each
+    use of the network cost no iterations, on every held-out design?"""
+    def every(a, b):
+        return all(r[a] <= r[b] for r in rows)
+
+    return {"deflated_net_le_zero": every("fp64_jacobi_deflated_net", "fp64_jacobi_zero"),
+            "deflated_net_prev_le_deflated_prev": every("fp64_jacobi_deflated_net_prev", "fp64_jacobi_deflated_prev"),
+            "net_start_le_zero": every("fp64_jacobi_net_start", "fp64_jacobi_zero"),
+            "multigrid_net_start_le_zero": every("fp64_multigrid_net_start", "fp64_multigrid_zero")}
 
 
 def main(argv=None) -> int:
@@ -182,7 +197,7 @@ def main(argv=None) -> int:
     ap.add_argument("--iters", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--batch", type=int, default=4)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=None, help="Adam learning rate (default 1e-3; 3e-4 for --study)")
     ap.add_argument("--channels", type=int, default=16)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--mode", default="load", choices=["load", "shape", "basis", "update"])
@@ -199,7 +214,8 @@ def main(argv=None) -> int:
         from . import workloads as wl
         tiny = tuple(int(v) for v in a.tiny.split(",")) if a.tiny else None
         res = study_config3(wl.make(a.config, tiny=tiny), a.traj, a.iters, a.epochs, a.channels, seed=a.seed,
-                            mode=a.mode, log=lambda s: print(s, flush=True))
+                            mode=a.mode, lr=a.lr if a.lr is not None else 3e-4, batch=a.batch,
+                            log=lambda s: print(s, flush=True))
         with open(a.out, "w") as fh:
             json.dump(res, fh, indent=1)
         print(json.dumps({"study": a.out, "mean": res["mean"]}), flush=True)
@@ -212,7 +228,8 @@ def main(argv=None) -> int:
     tiny = tuple(int(v) for v in a.tiny.split(",")) if a.tiny else None
     base = wl.make(a.config, tiny=tiny)
     t0 = time.perf_counter()
-    net, c0, hist, held = train(base, a.traj, a.iters, a.epochs, batch=a.batch, lr=a.lr, base_channels=a.channels,
+    lr = a.lr if a.lr is not None else 1e-3
+    net, c0, hist, held = train(base, a.traj, a.iters, a.epochs, batch=a.batch, lr=lr, base_channels=a.channels,
                             seed=a.seed, mode=a.mode, log=log)
     log(f"training done in {time.perf_counter() - t0:.1f} s")
     idx = np.linspace(0, len(held) - 1, min(8, len(held))).astype(int)

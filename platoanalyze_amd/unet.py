@@ -15,6 +15,15 @@ from . import _capi
 IN_CH = 8
 OUT_CH = 16
 
+# (cin, cout, input level, output level) of the 15 convolutions in launch order, as
+# pa_unet.hip's build_layers defines them for base width C (level l = grid halved l times,
+# rounded up; stride-2 layers go down a level, transposed ones up).
+def conv_layers(C: int) -> list[tuple[int, int, int, int]]:
+    return [(IN_CH, C, 0, 0), (C, C, 0, 0), (C, 2 * C, 0, 1), (2 * C, 2 * C, 1, 1), (2 * C, 4 * C, 1, 2),
+            (4 * C, 4 * C, 2, 2), (4 * C, 8 * C, 2, 3), (8 * C, 8 * C, 3, 3), (8 * C, 4 * C, 3, 2),
+            (8 * C, 4 * C, 2, 2), (4 * C, 2 * C, 2, 1), (4 * C, 2 * C, 1, 1), (2 * C, C, 1, 0),
+            (2 * C, C, 0, 0), (C, OUT_CH, 0, 0)]
+
 
 def _p(t: torch.Tensor):
     if not t.is_cuda or t.dtype != torch.float32 or not t.is_contiguous():
@@ -50,6 +59,15 @@ class UNet:
         if x.shape[1:] != (*self.grid, IN_CH):
             raise ValueError(f"input must be [batch, {self.grid}, {IN_CH}], got {tuple(x.shape)}")
         return int(x.shape[0])
+
+    def forward_flops(self, batch: int) -> int:
+        """FLOPs of one forward pass as the implicit-GEMM convolutions compute them:
+        2 x (batch x output voxels) x cout x 27 cin per layer (all 27 taps, padding included)."""
+        g = [self.grid]
+        for _ in range(3):
+            g.append(tuple((n + 1) // 2 for n in g[-1]))
+        vox = [a * b * c for a, b, c in g]
+        return sum(2 * batch * vox[lo] * co * 27 * ci for ci, co, _, lo in conv_layers(self.base))
 
     def params(self) -> torch.Tensor:
         out = torch.empty(self.num_params, dtype=torch.float32, device="cuda")

@@ -221,6 +221,21 @@ class WarmStart:
         return galerkin_scale(self.model, z, u_hat, f, self.bc) * u_hat
 
 
+    def deflation_space(self, z: torch.Tensor, f: torch.Tensor | None = None,
+                        u_prev: torch.Tensor | None = None) -> torch.Tensor:
+        """Vectors for Problem.solution_deflated (k x 3N): the network's guess and, in an
+        optimisation loop, the previous design's solution.  Deflation takes the best
+        combination of them (a Galerkin start) and keeps the iteration orthogonal to
+        them, so a poor guess costs little where as a starting point it can cost
+        iterations."""
+        vecs = [self.initial_guess(z, f)] if self.mode != "update" else []
+        if u_prev is not None:
+            vecs.append(u_prev)
+        if not vecs:
+            raise ValueError("update mode: pass u_prev (the guess needs the previous design)")
+        return torch.stack(vecs).contiguous()
+
+
 def save(net: UNet, c0: float, path: str | Path, meta: dict) -> None:
     p = net.params().cpu().numpy()
     np.savez(path, params=p, grid=np.array(net.grid), base=np.array(net.base), c0=np.array(c0),

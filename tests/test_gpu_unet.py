@@ -37,7 +37,10 @@ def test_gpu_unet_forward_matches_oracle():
     out = net.forward(xt).reshape(2, -1, U.OUT_CH).cpu().numpy()
     err = _rel(out, out_ref)
     print(f"forward rel L2 error {err:.2e}", flush=True)
-    assert err < 1e-2
+    # measured 1.2e-4 to 2.8e-4 (BF16 operands rounded at the same points as the oracle;
+    # the FP32 accumulation order of the MFMA tiles and of the split reductions moves a
+    # few activations across a BF16 rounding boundary, one BF16 ulp = 3.9e-3 relative)
+    assert err < 2e-3
 
 
 def test_gpu_unet_gradient_matches_oracle():
@@ -52,14 +55,16 @@ def test_gpu_unet_gradient_matches_oracle():
     g = g.cpu().numpy()
     err = _rel(g, g_ref)
     print(f"loss {loss:.6e} ref {loss_ref:.6e}; gradient rel L2 error {err:.2e}", flush=True)
-    assert abs(loss - loss_ref) <= 1e-2 * abs(loss_ref)
-    assert err < 3e-2
+    # measured: loss to 6e-7, gradient 6.1e-8 to 2.4e-4 (the same BF16 rounding flips of
+    # activations as in the forward check)
+    assert abs(loss - loss_ref) <= 1e-5 * abs(loss_ref)
+    assert err < 2e-3
     # per layer, so a wrong layer cannot hide behind the large ones
     offs = np.cumsum([0] + [27 * l[2] * l[5] + l[5] for l in ref.L])
-    for k in range(len(ref.L)):
-        a, b = g[offs[k]:offs[k + 1]], g_ref[offs[k]:offs[k + 1]]
-        if np.linalg.norm(b) > 0:
-            assert _rel(a, b) < 5e-2, (k, _rel(a, b))
+    per_layer = [_rel(g[offs[k]:offs[k + 1]], g_ref[offs[k]:offs[k + 1]]) for k in range(len(ref.L))
+                 if np.linalg.norm(g_ref[offs[k]:offs[k + 1]]) > 0]
+    print(f"largest per-layer gradient rel L2 error {max(per_layer):.2e}", flush=True)
+    assert max(per_layer) < 1e-2
 
 
 def test_gpu_unet_training_reduces_loss():
@@ -78,7 +83,9 @@ def test_gpu_warm_start_end_to_end():
     """Train a small U-Net on SIMP OC trajectories of a 16x8x8 box (library solves and
     kernels end to end), then solve held-out designs from zero and from the network's
     warm start.  Both must reach the same solution on the true FP64 residual; the
-    iteration counts of both are printed: that difference is what the network gains."""
+    iteration counts of both are printed: that difference is what the network gains.
+    Deflating FP64 block-Jacobi PCG by the network's guess must not cost iterations
+    (deterministic counts) and must reach the same solution."""
     from platoanalyze_amd import workloads as wl
     from platoanalyze_amd.elliptic import Problem
     from platoanalyze_amd.warmstart import WarmStart, train
@@ -88,18 +95,27 @@ def test_gpu_warm_start_end_to_end():
     net, c0, hist, held = train(base, 4, 6, epochs=20, batch=4, lr=2e-3, base_channels=8, seed=3,
                                 log=lambda s: None)
     assert hist[-1] < hist[0]  # the library's own kernels trained the network
-    cold, warm = [], []
+    cold, warm, plain, defl = [], [], [], []
     for w, z, f in held[::2]:
         m = wl.model(w)
         pb = Problem(m, tolerance=1e-9, check_every=8, precision="mixed")
-        u_c = pb.solution(z)
+        u_c = pb.solution(z).clone()
         cold.append(pb.stats.iterations)
-        u0 = WarmStart(net, w, m, c0).initial_guess(z, f)
+        ws = WarmStart(net, w, m, c0)
+        u0 = ws.initial_guess(z, f)
         u_w = pb.solution(z, u0=u0)
         warm.append(pb.stats.iterations)
         assert pb.stats.converged and pb.stats.rel_residual <= 1e-9
         assert float((u_w - u_c).norm() / u_c.norm()) < 1e-6
-    print(f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; iterations cold {cold} warm {warm}", flush=True)
+        pj = Problem(m, tolerance=1e-9, check_every=1)
+        pj.solution(z)
+        plain.append(pj.stats.iterations)
+        u_d = pj.solution_deflated(z, ws.deflation_space(z, f))
+        defl.append(pj.stats.iterations)
+        assert pj.stats.converged and float((u_d - u_c).norm() / u_c.norm()) < 1e-6
+    print(f"loss {hist[0]:.3e} -> {hist[-1]:.3e}; iterations cold {cold} warm {warm}; "
+          f"FP64 Jacobi plain {plain} deflated by the guess {defl}", flush=True)
+    assert all(d <= p for d, p in zip(defl, plain))
 
 
 def test_gpu_warm_start_update_mode():

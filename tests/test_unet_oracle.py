@@ -71,3 +71,20 @@ def test_backward_matches_central_differences():
             assert abs(fd - g[i]) <= 1e-6 * np.abs(g).max() + 1e-5 * abs(fd), (k, i, fd, g[i])
             checked += 1
     assert checked == 3 * len(r.L)
+
+
+def test_forward_flops_counts_every_oracle_link_slot():
+    """UNet.forward_flops (the FLOP count bench.py reports) equals 2 x 27 x cin x cout x
+    output voxels summed over the oracle's layer table and grid levels."""
+    from platoanalyze_amd.unet import UNet, conv_layers
+
+    grid, C, batch = (65, 33, 33), 16, 4
+    r = U.UNetRef(grid, C)
+    want = 0
+    for (bi, _, cin, bo, _, cout, *_rest) in r.L:
+        go = r.g[U.BUF_LEVEL[bo]] if bo >= 0 else r.g[0]
+        want += 2 * batch * int(np.prod(go)) * cout * 27 * cin
+    assert [(l[2], l[5], U.BUF_LEVEL[l[0]], U.BUF_LEVEL[l[3]] if l[3] >= 0 else 0) for l in r.L] == conv_layers(C)
+    net = UNet.__new__(UNet)  # no device handle: only the geometry is used
+    net.grid, net.base = grid, C
+    assert net.forward_flops(batch) == want
