@@ -242,8 +242,15 @@ def main() -> int:
 
     box = C.c_int(0)
     _capi.call("pa_box_assembly", h, C.byref(box))
-    kernel_name = ("k_box_scatter + k_box_w + k_box_jac_ws<true> (lattice Jacobian+residual)" if box.value
-                   else "k_jac_chunk<4,true> (Jacobian+residual)")
+    # the library takes k_box_jac_ws_ahead when the lattice weights and u (72 B per padded
+    # hex) exceed 256 MB, unless PA_BOX_AHEAD says otherwise
+    c = w.cfg
+    ahead = 72 * (c.nx + 2) * (c.ny + 2) * (c.nz + 2) > (256 << 20)
+    if os.environ.get("PA_BOX_AHEAD"):
+        ahead = os.environ["PA_BOX_AHEAD"] != "0"
+    kernel_name = ("k_jac_chunk<4,true> (Jacobian+residual)" if not box.value else
+                   "k_box_scatter + k_box_w + k_box_jac_ws_ahead<true> + k_add_load (lattice Jacobian+residual)"
+                   if ahead else "k_box_scatter + k_box_w + k_box_jac_ws<true> (lattice Jacobian+residual)")
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):

@@ -259,16 +259,16 @@ template __global__ void k_box_jac<false>(BoxArgs);
 template __global__ void k_box_jac<true>(BoxArgs);
 
 template <bool WITH_R>
-__global__ __launch_bounds__(512, 4) void k_box_jac_ws(BoxArgs A) {
+__global__ __launch_bounds__(kBoxWsThreads, 2) void k_box_jac_ws(BoxArgs A) {
   // LDS: two tile buffers of CRS values, two of residual rows, the block table, and the
   // CRS range [s_base, s_end) and first row of the tile in each buffer.
   extern __shared__ double dyn[];
   __shared__ double sB[16 * kBoxMaxTerms * 9];
   __shared__ int hdr[2][3];
   const int cap = A.seg_cap;
-  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += 512) sB[q] = A.B[q];
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += kBoxWsThreads) sB[q] = A.B[q];
   const bool producer = threadIdx.x < 256;
-  const int tid = threadIdx.x & 255;
+  const int tid = producer ? (int)threadIdx.x : (int)threadIdx.x - 256;
   const int j = tid % 8, grp = tid / 8;
   const int la = j, lb = j + 8;
   const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(512, 4) void k_box_jac_ws(BoxArgs A) {
       const int n = 9 * (se - sb);
       const int head = sb & 1;
       if (head && tid == 0 && n > 0) __builtin_nontemporal_store(seg[0], out);
-      for (int q = head + 2 * tid; q + 1 < n; q += 512)
+      for (int q = head + 2 * tid; q + 1 < n; q += 2 * (kBoxWsThreads - 256))
         __builtin_nontemporal_store(v2d{seg[q], seg[q + 1]}, reinterpret_cast<v2d*>(out + q));
       if (n > head && ((n - head) & 1) && tid == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
       if constexpr (WITH_R) {
@@ -426,6 +426,222 @@ __global__ __launch_bounds__(512, 4) void k_box_jac_ws(BoxArgs A) {
 }
 template __global__ void k_box_jac_ws<false>(BoxArgs);
 template __global__ void k_box_jac_ws<true>(BoxArgs);
+
+template <bool WITH_R>
+__global__ __launch_bounds__(kBoxWsThreads, 2) void k_box_jac_ws_ahead(BoxArgs A) {
+  // LDS: two tile buffers of CRS values, two of residual rows, the block table, and the
+  // CRS range [s_base, s_end) and first row of the tile in each buffer.
+  extern __shared__ double dyn[];
+  __shared__ double sB[16 * kBoxMaxTerms * 9];
+  __shared__ int hdr[2][3];
+  const int cap = A.seg_cap;
+  for (int q = threadIdx.x; q < 16 * kBoxMaxTerms * 9; q += kBoxWsThreads) sB[q] = A.B[q];
+  // four computing waves (threads 0..255, 8 lanes per row of a 32-row tile) and two
+  // streaming waves (threads 256.., ctid 0..127)
+  const bool producer = threadIdx.x < 256;
+  const int tid = threadIdx.x & 255, ctid = (int)threadIdx.x - 256;
+  const int j = tid % 8, grp = tid / 8;
+  const int la = j, lb = j + 8;
+  const int cnta = A.S->count[la], cntb = lb == kBoxDiagLane ? 0 : A.S->count[lb];
+  unsigned offa[kBoxMaxTerms], offb[kBoxMaxTerms];
+#pragma unroll
+  for (int e = 0; e < kBoxMaxTerms; ++e) {
+    offa[e] = (unsigned)A.S->off[la][e];
+    offb[e] = (unsigned)A.S->off[lb][e];
+  }
+  const int da0 = A.S->d[la][0], da1 = A.S->d[la][1], da2 = A.S->d[la][2];
+  const int db0 = A.S->d[lb][0], db1 = A.S->d[lb][1], db2 = A.S->d[lb][2];
+  const unsigned uoffa = (unsigned)A.S->uoff[la], uoffb = (unsigned)A.S->uoff[lb];
+  const unsigned px = A.nx + 2, py = A.ny + 2;
+  const double* __restrict__ W = A.W;
+  const double* __restrict__ ul = A.ulat;
+  const double* Ba = sB + la * (kBoxMaxTerms * 9);
+  const double* Bb = sB + lb * (kBoxMaxTerms * 9);
+  const int nblk = (A.nrows + kBoxRows - 1) / kBoxRows;
+  const int per = (nblk + kNumXcd - 1) / kNumXcd;
+  const int x = blockIdx.x % kNumXcd, nbx = gridDim.x / kNumXcd;
+  const int t0 = x * per + (int)blockIdx.x / kNumXcd, t_end = min((x + 1) * per, nblk);
+  const int ntile = t0 < t_end ? (t_end - t0 + nbx - 1) / nbx : 0;
+  const int i0 = rs_first(j);  // first of this lane's two entries of the row sums
+  // The computing waves' loads are all unconditional (clamped rows, selects instead of
+  // branches), so the compiler's count of loads issued after a tile's gathers is exact:
+  // a tile's gathers are issued right after the previous tile's arithmetic and are
+  // waited for one staging and one hand-over later.
+  struct Rows {
+    int sb, se, s0, gr;
+    unsigned long long pm;
+  };
+  struct Gath {
+    double wa[kBoxMaxTerms], wb[kBoxMaxTerms], ur[3], ua[3], ub[3];
+  };
+  // row data of tile kk (clamped to the last tile; a row past the end of its tile reads
+  // the tile's last row and is masked where it matters)
+  auto load_rows = [&](int kk, Rows& r) {
+    const int base = (t0 + min(kk, ntile - 1) * nbx) * kBoxRows;
+    const int last = min(base + kBoxRows, A.nrows);
+    const int row = min(base + grp, last - 1);
+    r.sb = A.rowmap[base];
+    r.se = A.rowmap[last];
+    r.s0 = A.rowmap[row];
+    r.gr = A.gpack[row];
+    r.pm = A.perm[row];
+  };
+  // in-bounds gathers for every lane: terms a lane does not have point at offset 0 with a
+  // zero block, ghost hexes weigh zero, a lane without a neighbour reads the row's own u
+  auto gather = [&](const Rows& r, Gath& g) {
+    const int ri = r.gr & 1023, rj = (r.gr >> 10) & 1023, rk = r.gr >> 20;
+    const bool ina = cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx && (unsigned)(rj + da1) <= (unsigned)A.ny &&
+                     (unsigned)(rk + da2) <= (unsigned)A.nz;
+    const bool inb = cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx && (unsigned)(rj + db1) <= (unsigned)A.ny &&
+                     (unsigned)(rk + db2) <= (unsigned)A.nz;
+    const unsigned hr = (unsigned)(ri + 1) + px * ((unsigned)(rj + 1) + py * (unsigned)(rk + 1));
+#pragma unroll
+    for (int e = 0; e < kBoxMaxTerms; ++e) {
+      g.wa[e] = W[6 * hr + offa[e]];
+      g.wb[e] = W[6 * hr + offb[e]];
+    }
+    if constexpr (WITH_R) {
+      const unsigned hca = ina ? hr + uoffa : hr, hcb = inb ? hr + uoffb : hr;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        g.ur[i] = ul[3 * hr + i];
+        g.ua[i] = ul[3 * hca + i];
+        g.ub[i] = ul[3 * hcb + i];
+      }
+    }
+  };
+  // row data of tiles k, k + 1 and k + 2 rotate through three sets (the loop below is
+  // unrolled by three), so no loaded register is copied, which would wait for its load
+  Rows rA{}, rB{}, rC{};
+  Gath g{};
+  if (producer && ntile > 0) {
+    load_rows(0, rA);
+    load_rows(1, rB);
+    gather(rA, g);
+  }
+  __syncthreads();  // block table in LDS
+  // iteration k: the producers compute tile k into buffer k & 1 while the consumers
+  // stream tile k - 1 out of the other buffer; one barrier per iteration hands over
+  auto iteration = [&](int k, const Rows& rc, const Rows& rn, Rows& r2) __attribute__((always_inline)) {
+    const int b = k & 1;
+    if (producer) {
+      if (k < ntile) {
+        double* seg = dyn + (size_t)b * 9 * cap;
+        double* rseg = dyn + (size_t)2 * 9 * cap + b * 3 * kBoxRows;
+        const int base = (t0 + k * nbx) * kBoxRows;
+        const bool live = base + grp < min(base + kBoxRows, A.nrows);
+        const int ri = rc.gr & 1023, rj = (rc.gr >> 10) & 1023, rk = rc.gr >> 20;
+        const bool nba = live && cnta > 0 && (unsigned)(ri + da0) <= (unsigned)A.nx &&
+                         (unsigned)(rj + da1) <= (unsigned)A.ny && (unsigned)(rk + da2) <= (unsigned)A.nz;
+        const bool nbb = live && cntb > 0 && (unsigned)(ri + db0) <= (unsigned)A.nx &&
+                         (unsigned)(rj + db1) <= (unsigned)A.ny && (unsigned)(rk + db2) <= (unsigned)A.nz;
+        double dua[3] = {0.0, 0.0, 0.0}, dub[3] = {0.0, 0.0, 0.0};
+        if constexpr (WITH_R) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            dua[i] = g.ua[i] - g.ur[i];
+            dub[i] = g.ub[i] - g.ur[i];
+          }
+        }
+        double Ka[9], Kb[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          Ka[q] = g.wa[0] * Ba[q];
+          Kb[q] = g.wb[0] * Bb[q];
+        }
+#pragma unroll
+        for (int e = 1; e < kBoxMaxTerms; ++e)
+#pragma unroll
+          for (int q = 0; q < 9; ++q) {
+            Ka[q] = fma(g.wa[e], Ba[9 * e + q], Ka[q]);
+            Kb[q] = fma(g.wb[e], Bb[9 * e + q], Kb[q]);
+          }
+        double v12[12];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) v12[q] = Ka[q] + Kb[q];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+          v12[9 + i] = Ka[3 * i] * dua[0] + Ka[3 * i + 1] * dua[1] + Ka[3 * i + 2] * dua[2] +
+                       Kb[3 * i] * dub[0] + Kb[3 * i + 1] * dub[1] + Kb[3 * i + 2] * dub[2];
+        double sum0, sum1;
+        reduce_scatter12x8(v12, j, sum0, sum1);
+        // the next tiles' loads go out now: row data two tiles ahead, gathers one ahead
+        load_rows(k + 2, r2);
+        gather(rn, g);
+        if (tid == 0) {
+          hdr[b][0] = rc.sb;
+          hdr[b][1] = rc.se;
+          hdr[b][2] = base;
+        }
+        if (live) {
+          double* dg = seg + 9 * (rc.s0 - rc.sb + (int)((rc.pm >> (4 * kBoxDiagLane)) & 15));
+          if (i0 < 9) dg[i0] = -sum0;
+          if ((j & 1) == 0 && i0 + 1 < 9) dg[i0 + 1] = -sum1;
+          if constexpr (WITH_R) {
+            if (i0 >= 9) rseg[3 * grp + i0 - 9] = sum0;
+            if ((j & 1) == 0 && i0 + 1 >= 9) rseg[3 * grp + i0 - 8] = sum1;
+          }
+        }
+        if (nba) {
+          double* d = seg + 9 * (rc.s0 - rc.sb + (int)((rc.pm >> (4 * la)) & 15));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) d[q] = Ka[q];
+        }
+        if (nbb) {
+          double* d = seg + 9 * (rc.s0 - rc.sb + (int)((rc.pm >> (4 * lb)) & 15));
+#pragma unroll
+          for (int q = 0; q < 9; ++q) d[q] = Kb[q];
+        }
+      }
+    } else if (k > 0) {
+      // stream tile k - 1: CRS values with 16-byte non-temporal stores from the first
+      // 16-byte-aligned value on, then its residual rows (base is a multiple of 32, so
+      // 3 base doubles start on a 16-byte boundary)
+      typedef double v2d __attribute__((ext_vector_type(2)));
+      const double* seg = dyn + (size_t)(b ^ 1) * 9 * cap;
+      const int sb = hdr[b ^ 1][0], se = hdr[b ^ 1][1], base = hdr[b ^ 1][2];
+      double* out = A.vals + 9 * (size_t)sb;
+      const int n = 9 * (se - sb);
+      const int head = sb & 1;
+      constexpr int kCons = kBoxWsThreads - 256, kU = 4;
+      if (head && ctid == 0 && n > 0) __builtin_nontemporal_store(seg[0], out);
+      // kU pairs per thread read from LDS back to back, then stored
+      for (int q0 = head + 2 * ctid; q0 + 1 < n; q0 += 2 * kCons * kU) {
+        v2d v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int q = q0 + 2 * kCons * u;
+          if (q + 1 < n) v[u] = v2d{seg[q], seg[q + 1]};
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int q = q0 + 2 * kCons * u;
+          if (q + 1 < n) __builtin_nontemporal_store(v[u], reinterpret_cast<v2d*>(out + q));
+        }
+      }
+      if (n > head && ((n - head) & 1) && ctid == 0) __builtin_nontemporal_store(seg[n - 1], out + n - 1);
+      if constexpr (WITH_R) {
+        static_assert(2 * kCons >= 3 * kBoxRows, "one residual pair per streaming lane");
+        const double* rseg = dyn + (size_t)2 * 9 * cap + (b ^ 1) * 3 * kBoxRows;
+        const int nr = 3 * (min(base + kBoxRows, A.nrows) - base);
+        double* rout = A.R + 3 * (size_t)base;
+        if (2 * ctid + 1 < nr)
+          __builtin_nontemporal_store(v2d{rseg[2 * ctid], rseg[2 * ctid + 1]},
+                                      reinterpret_cast<v2d*>(rout + 2 * ctid));
+        else if (2 * ctid < nr)
+          __builtin_nontemporal_store(rseg[2 * ctid], rout + 2 * ctid);
+      }
+    }
+    lds_barrier();
+  };
+  for (int k = 0; k <= ntile; k += 3) {
+    iteration(k, rA, rB, rC);
+    if (k + 1 <= ntile) iteration(k + 1, rB, rC, rA);
+    if (k + 2 <= ntile) iteration(k + 2, rC, rA, rB);
+  }
+}
+template __global__ void k_box_jac_ws_ahead<false>(BoxArgs);
+template __global__ void k_box_jac_ws_ahead<true>(BoxArgs);
 
 __global__ __launch_bounds__(256, 5) void k_box_res(BoxArgs A, int sub_load) {
   __shared__ double sB[16 * kBoxMaxTerms * 9];

@@ -76,11 +76,18 @@ template <bool WITH_R>
 __global__ void k_box_jac(BoxArgs A);
 
 // The same values with the work-group's waves specialised: four waves compute tile k+1
-// into one LDS buffer while four others stream tile k out of the other, so the waves
+// into one LDS buffer while two others stream tile k out of the other, so the waves
 // that wait on gathers never hold outstanding stores and the waves that store never
-// wait on loads.  512 threads; dynamic LDS of box_jac_ws_lds(seg_cap) bytes.
+// wait on loads.  kBoxWsThreads threads; dynamic LDS of box_jac_ws_lds(seg_cap) bytes.
+constexpr int kBoxWsThreads = 384;
 template <bool WITH_R>
 __global__ void k_box_jac_ws(BoxArgs A);
+// k_box_jac_ws with the computing waves' gathers issued one tile ahead (right after the
+// previous tile's arithmetic), for lattices whose weights and displacements do not fit
+// the last-level cache; R is written without the load (the caller subtracts f).  Same
+// threads and LDS.
+template <bool WITH_R>
+__global__ void k_box_jac_ws_ahead(BoxArgs A);
 inline size_t box_jac_ws_lds(int seg_cap) { return (size_t)(2 * 9 * seg_cap + 2 * 3 * kBoxRows) * sizeof(double); }
 
 // FP32 matrix-free operator on the lattice (the mixed-precision PCG's K p and the

@@ -95,6 +95,10 @@ struct DevArray {
 
 // One level of the geometric multigrid hierarchy of a box mesh.  Level 0 is the
 // problem itself; coarser levels own a problem handle on the coarsened box.
+// Lattice arrays (element weights and lattice u) above this size take k_box_jac_ws_ahead:
+// about the last-level cache (Infinity Cache) of the part.
+constexpr size_t kBoxAheadBytes = (size_t)256 << 20;
+
 struct MgLevel {
   pa_handle h = nullptr;  // owned for levels >= 1
   bool owned = false;
@@ -203,6 +207,9 @@ struct pa_problem {
   // k_box_jac_ws (store and compute waves specialised; PA_BOX_WS=0 selects k_box_jac)
   bool box_ws = true;
   int box_seg_cap = 0, box_grid_ws = 512;
+  // k_box_jac_ws_ahead in place of k_box_jac_ws: lattices whose element weights and
+  // lattice u exceed kBoxAheadBytes (PA_BOX_AHEAD=0/1 overrides)
+  bool box_ahead = false;
   bool box_res = true;  // lattice FP64 residual (k_box_res); PA_BOX_RES=0 keeps the chunk kernel
   int box_grid_res = 1024;
   double* host_scal = nullptr;
@@ -1440,7 +1447,9 @@ static int box_fast_setup(pa_handle h) {
     int dev = 0, cus = 256, bw = 2;
     // the dynamic LDS of the specialised kernel exceeds the default 64 KiB limit
     for (const void* f : {reinterpret_cast<const void*>(&pa::k_box_jac_ws<true>),
-                          reinterpret_cast<const void*>(&pa::k_box_jac_ws<false>)})
+                          reinterpret_cast<const void*>(&pa::k_box_jac_ws<false>),
+                          reinterpret_cast<const void*>(&pa::k_box_jac_ws_ahead<true>),
+                          reinterpret_cast<const void*>(&pa::k_box_jac_ws_ahead<false>)})
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
         (void)hipGetLastError();  // leave no error behind for the caller's next HIP call
         h->box_ws = false;
@@ -1448,11 +1457,18 @@ static int box_fast_setup(pa_handle h) {
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
       cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, pa::k_box_jac_ws<true>, 512, lds) != hipSuccess || bw < 1) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bw, pa::k_box_jac_ws<true>, pa::kBoxWsThreads, lds) != hipSuccess || bw < 1) {
       (void)hipGetLastError();
       bw = 1;
     }
     h->box_grid_ws = (bw * cus + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd;
+  }
+  {
+    // measured at 256x128x128 (314 MB of weights and u): 1.65 ms against 1.78 ms; at
+    // 128x64x64 (39 MB, cache-resident) the plain kernel is faster (0.17 against 0.21 ms)
+    const size_t lattice_bytes = (size_t)(6 + 3) * sizeof(double) * (hx + 2) * (hy + 2) * (hz + 2);
+    h->box_ahead = lattice_bytes > kBoxAheadBytes;
+    if (const char* e = std::getenv("PA_BOX_AHEAD")) h->box_ahead = std::atoi(e) != 0;
   }
   h->box_fast = true;
   return PA_OK;
@@ -1520,10 +1536,19 @@ static int box_jacobian(pa_handle h, const double* u, const double* z, double* v
     aw.seg_cap = h->box_seg_cap;
     const size_t lds = pa::box_jac_ws_lds(h->box_seg_cap);
     const int G = std::min(tiles8, h->box_grid_ws);
-    if (R)
-      pa::k_box_jac_ws<true><<<G, 512, lds, h->stream>>>(aw);
-    else
-      pa::k_box_jac_ws<false><<<G, 512, lds, h->stream>>>(aw);
+    if (h->box_ahead) {
+      if (R) {
+        pa::k_box_jac_ws_ahead<true><<<G, pa::kBoxWsThreads, lds, h->stream>>>(aw);
+        PA_HIP(hipGetLastError());
+        if (int rc = sub_load(h, R, -1.0)) return rc;  // the kernel writes R = K u
+      } else {
+        pa::k_box_jac_ws_ahead<false><<<G, pa::kBoxWsThreads, lds, h->stream>>>(aw);
+      }
+    } else if (R) {
+      pa::k_box_jac_ws<true><<<G, pa::kBoxWsThreads, lds, h->stream>>>(aw);
+    } else {
+      pa::k_box_jac_ws<false><<<G, pa::kBoxWsThreads, lds, h->stream>>>(aw);
+    }
   } else {
     const int G = std::min(tiles8, R ? h->box_grid_r : h->box_grid_k);
     if (R)

@@ -69,9 +69,13 @@ def close(got, ref, tol, what):
 
 @pytest.mark.parametrize("dims", [(4, 2, 2), (20, 10, 10), (7, 5, 3)])
 @pytest.mark.parametrize("penalty", [("SIMP", 3.0, 1e-3), ("SIMP", 2.5, 1e-3), ("RAMP", 2.0, 1e-2), ("Heaviside", 4.0, 0.0)])
-def test_gpu_box_matches_general_and_oracle(dims, penalty):
+@pytest.mark.parametrize("ahead", ["0", "1"])
+def test_gpu_box_matches_general_and_oracle(dims, penalty, ahead, monkeypatch):
+    """ahead = 1 forces the kernel that large lattices take (gathers one tile ahead, the
+    load subtracted after the kernel) onto these small ones."""
+    monkeypatch.setenv("PA_BOX_AHEAD", ahead)
     name, p, rmin = penalty
-    print(f"case: box assembly {dims} {name}", flush=True)
+    print(f"case: box assembly {dims} {name} ahead={ahead}", flush=True)
     mesh = build_box(*dims, 2.0, 1.0, 1.0)
     rng = np.random.default_rng(sum(dims))
     z = rng.uniform(0.05, 1.0, mesh.num_nodes)
