@@ -272,6 +272,25 @@ def main() -> int:
     B = asm_bytes(E, N, nnzb)
     achieved = B / (kern_ms * 1e-3) / 1e9
 
+    # measured ceilings on this box: a 1 GiB device copy (read + write bytes) and a 1 GiB
+    # fill (write bytes), HIP events over 20 launches each
+    def ceiling(fn, nbytes, reps=20):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9
+
+    src_c = torch.ones(1 << 27, dtype=torch.float64, device=dev)
+    dst_c = torch.empty_like(src_c)
+    copy_gbs = ceiling(lambda: dst_c.copy_(src_c), 2 * src_c.numel() * 8)
+    fill_gbs = ceiling(lambda: dst_c.fill_(1.0), dst_c.numel() * 8)
+    del src_c, dst_c
+
     traffic = None
     tf = ROOT / "profiles" / f"traffic_{args.config}.json"
     if tf.exists():
@@ -377,7 +396,9 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel_name, "kernel_ms": kern_ms,
-                     "algorithmic_bytes": B},
+                     "algorithmic_bytes": B,
+                     "measured_copy_GBs": copy_gbs, "measured_fill_GBs": fill_gbs,
+                     "frac_of_measured_copy": achieved / copy_gbs},
         "cpu_baseline": cpu,
         "setup_s": {"mesh": t_mesh, "graph_chunks_upload": t_setup},
     }
