@@ -987,6 +987,7 @@ static int mixed_setup(pa_handle h, const double* z) {
   da.bcmask = h->bcmask.p;
   da.dinv = h->m_dinv.p;
   da.m = h->material();
+  da.gbound = h->chunk_gbound.p;
   if (h->box_fast && h->box_mf) {
     // the lattice FP32 operator (element weights of this density in FP32) and the block
     // diagonal from the FP64 weights, both summed in a fixed order
@@ -1076,6 +1077,7 @@ static int solve_mixed(pa_handle h, const double* z, double* u, const double* b,
   ma.scal = h->scal.p;
   ma.partials = h->chunk_partials.p;
   ma.m = h->material();
+  ma.gbound = h->chunk_gbound.p;
   // r = mask(b - R) into m_r and ||r||^2
   auto masked_residual = [&](const double* R, double* out) -> int {
     PA_HIP(pa::launch_neg_mask(R, b, h->bcmask.p, h->m_r.p, nn, PS(pa::kPDot), G, s));
@@ -1690,6 +1692,7 @@ static int mg_apply(pa_handle h, size_t l, const double* x, double* y, const dou
       ma.y = y;
       ma.partials = h->chunk_partials.p;
       ma.m = h->material();
+      ma.gbound = h->chunk_gbound.p;
       const double* Pq = nullptr;
       int npq = 0;
       if (int rc = mf_apply(h, ma, &Pq, &npq)) return rc;

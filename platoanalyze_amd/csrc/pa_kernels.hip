@@ -3,6 +3,7 @@
 // (atomic-free in HBM: each CRS value and residual entry is written exactly once,
 // coalesced), then streams the finished segment out.
 #include "pa_kernels.hpp"
+#include "pa_fixed.hpp"
 #include "pa_dpp.hpp"
 
 namespace pa {
@@ -22,15 +23,7 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Order-independent LDS accumulation: v is rounded once to a multiple of 1/S (S a power
-// of two) and added as a 64-bit integer, so the sum does not depend on the order in
-// which lanes and waves arrive (integer addition is associative).  The slot holds the
-// integer until fixed_to_double converts it.
-__device__ __forceinline__ void lds_add_fixed(double* p, double v, double S) {
-  atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)llrint(v * S));
-}
-
-// Scale of a chunk's fixed-point accumulation: the largest power of two S with
+// Scale of a chunk's fixed-point Jacobian accumulation (lds_add_fixed64): the largest power of two S with
 // S * bound < 2^57, bound = gbound (|lambda| + 2 |mu|) on every element block entry
 // (weights <= 1), so up to 64 such contributions per entry cannot overflow and each is
 // resolved to 2^-57 of the bound.
@@ -119,7 +112,8 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   constexpr bool kReduce = MODE == kModeEnergy || kGradZ || kPnV || kVol;
   constexpr int kBlkW = kGzT ? 3 : (kGx ? 9 : 0);                      // doubles per block in LDS
   constexpr int kRowW = (kRes || kPnG || kEGx || kVol || kPnX) ? 3 : ((kGradZ || kAdj) ? 1 : 0);  // doubles per row
-  constexpr int kRow2 = (kPnG || kVol) ? 1 : 0;                        // second per-row output
+  // second per-row output (the residual: the low words of its two-word fixed-point sums)
+  constexpr int kRow2 = kRes ? 3 : ((kPnG || kVol) ? 1 : 0);
 
   const int c = xcd_chunk(blockIdx.x, A.nchunks);
   const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1];
@@ -135,12 +129,20 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   double* res2 = res + (size_t)A.max_rows * kRowW;
   int* roff = reinterpret_cast<int*>(res2 + (size_t)A.max_rows * kRow2);
 
+  // the residual sums in fixed point (order-independent): the scale comes from the
+  // largest |u| of the chunk's nodes, a NaN among them poisons it
+  __shared__ double red[NT / 64];
+  double umax = 0.0;
   for (int k = threadIdx.x; k < nnode; k += NT) {
     const size_t gid = (size_t)A.nodes[n0 + k];
     nodes[k] = make_double4(A.X[3 * gid], A.X[3 * gid + 1], A.X[3 * gid + 2], A.z[gid]);
     ul[3 * k + 0] = A.u[3 * gid + 0];
     ul[3 * k + 1] = A.u[3 * gid + 1];
     ul[3 * k + 2] = A.u[3 * gid + 2];
+    if constexpr (kRes) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) umax = max_nan(umax, fabs(ul[3 * k + i]));
+    }
     if constexpr (kAdj) {
       ul2[3 * k + 0] = A.u2[3 * gid + 0];
       ul2[3 * k + 1] = A.u2[3 * gid + 1];
@@ -155,7 +157,14 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     for (int k = threadIdx.x; k < nr * kRowW; k += NT) res[k] = 0.0;
   if constexpr (kRow2 > 0)
     for (int k = threadIdx.x; k < nr * kRow2; k += NT) res2[k] = 0.0;
+  if constexpr (kRes) block_max_write<NT>(umax, red);
   __syncthreads();
+  // |contribution| <= vol |grad N|^2 * 3 * (|c00| + 2|c01| + |c33|) * 8 max|u| for weights <= 1
+  double S = 1.0;
+  if constexpr (kRes) {
+    const double cs = fabs(A.m.c00) + 2.0 * fabs(A.m.c01) + fabs(A.m.c33);
+    S = fixed_scale_54(24.0 * cs * (double)A.gbound[c] * block_max_read<NT>(red));
+  }
 
   double energy = 0.0;
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
@@ -305,9 +314,10 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
           double r[3];
           tet_divergence(sw, g[a], vol, r);
           double* dst = &res[3 * ln[a]];
-          lds_add(dst + 0, r[0]);
-          lds_add(dst + 1, r[1]);
-          lds_add(dst + 2, r[2]);
+          double* lo = &res2[3 * ln[a]];
+          lds_add_fixed128(dst + 0, lo + 0, r[0], S);
+          lds_add_fixed128(dst + 1, lo + 1, r[1], S);
+          lds_add_fixed128(dst + 2, lo + 2, r[2], S);
         }
       }
     }
@@ -454,6 +464,9 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
   }
   __syncthreads();
   if constexpr (kRes) {
+    const double inv = 1.0 / S;  // a power of two: exact (NaN for a non-finite u)
+    for (int k = threadIdx.x; k < nr * 3; k += NT) res[k] = S == S ? fixed_value2(res[k], res2[k], inv) : NAN;
+    __syncthreads();
     if (A.sub_load) {
       for (int k = A.load_offs[c] + threadIdx.x; k < A.load_offs[c + 1]; k += NT) {
         const int lr = A.load_row[k];
@@ -472,7 +485,7 @@ __global__ __launch_bounds__(NT) void k_chunk(ChunkArgs A) {
     double* dst = A.out_vec + (size_t)r0 * kRowW;
     for (int k = threadIdx.x; k < nr * kRowW; k += NT) dst[k] = res[k];
   }
-  if constexpr (kRow2 > 0) {
+  if constexpr (kPnG || kVol) {
     double* dst = A.out_vec2 + (size_t)r0;
     for (int k = threadIdx.x; k < nr; k += NT) dst[k] = res2[k];
   }
@@ -637,7 +650,7 @@ __device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk&
 #pragma unroll
       for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) lds_add_fixed(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j], S);
+        for (int j = 0; j < 3; ++j) lds_add_fixed64(dst + 3 * i + j, rowb ? kv[j][i] : kv[i][j], S);
     }
   }
 }
@@ -646,7 +659,7 @@ __device__ __forceinline__ void jac_elements(const ChunkArgs& A, const JacChunk&
 template <int NT>
 __device__ __forceinline__ void jac_fixed_to_double(double* seg, int n, double S) {
   const double inv = 1.0 / S;  // a power of two: exact
-  for (int k = threadIdx.x; k < n; k += NT) seg[k] = (double)__double_as_longlong(seg[k]) * inv;
+  for (int k = threadIdx.x; k < n; k += NT) seg[k] = fixed_value(seg[k], inv);
 }
 
 // Phase 3 (after the element barrier): row groups finish the rows.
@@ -770,7 +783,7 @@ size_t chunk_lds_bytes(int mode, int max_rows, int max_blocks, int max_nodes) {
                     mode == kModePNormGradX)
                        ? 3
                        : ((mode == kModeEnergyGradZ || mode == kModeAdjZ) ? 1 : 0);
-  const int row2 = (mode == kModePNormGrad || mode == kModeVolume) ? 1 : 0;
+  const int row2 = mode == kModeResidual ? 3 : ((mode == kModePNormGrad || mode == kModeVolume) ? 1 : 0);
   const int nodew = mode == kModeAdjZ ? 10 : 7;
   size_t bytes = sizeof(double) * (nodew * (size_t)max_nodes + (size_t)max_blocks * blkw +
                                    (size_t)max_rows * (roww + row2));

@@ -1,6 +1,7 @@
 // Mixed-precision solve: FP32 matrix-free operator inside an FP64 Jacobi-PCG (FP64
 // vectors and reductions, residual replaced by the true FP64 residual f - K u).
 #include "pa_mf.hpp"
+#include "pa_fixed.hpp"
 #include "pa_reduce.hpp"
 #include "pa_solver.hpp"
 
@@ -51,7 +52,8 @@ __device__ __forceinline__ void gradients_t(const T x[4][3], T g[4][3], T& vol) 
 // vectors are FP64; the element operator is FP32: the node table (FP32 coordinates +
 // density) and p = z + beta p_prev at the table's nodes (rounded to FP32, constrained
 // dofs 0) are staged in LDS, the element loop computes K_e p_e in FP32 and adds it into
-// FP64 row accumulators with ds_add_f64, and the rows are written back with the
+// 64-bit fixed-point row accumulators (scale from the chunk's largest |p|: the sums do
+// not depend on the order of the adds), and the rows are written back with the
 // chunk's p.q partial.
 // The chunk's own rows of p_cur are written in FP64 (unrounded).
 template <int NT>
@@ -66,11 +68,13 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
   float4* nd4 = reinterpret_cast<float4*>(acc + 3 * mr2);
   float* xl = reinterpret_cast<float*>(nd4 + A.max_nodes);
   __shared__ double sh[NT / 64];
+  __shared__ double red[NT / 64];
 
   const double beta = A.first ? 0.0 : A.scal[kBeta];
   const int c = xcd_map(blockIdx.x, A.nchunks);
   const int r0 = A.chunk_rows[c], r1 = A.chunk_rows[c + 1], nr = r1 - r0;
   const int n0 = A.node_offs[c], nnode = A.node_offs[c + 1] - n0;
+  double pmax = 0.0;
   for (int k = threadIdx.x; k < nnode; k += NT) {
     const int gid = A.nodes[n0 + k];
     nd4[k] = A.Xz[gid];
@@ -81,6 +85,7 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
       double v = A.first ? A.z[d] : fma(beta, A.p_prev[d], A.z[d]);
       v = (bm >> i) & 1u ? 0.0 : v;
       xl[3 * k + i] = (float)v;
+      pmax = max_nan(pmax, fabs((double)(float)v));
       if (k < nr) {  // rows come first in the table
         A.p_cur[d] = v;
         pown[3 * k + i] = v;
@@ -88,7 +93,12 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
     }
   }
   for (int k = threadIdx.x; k < 3 * nr; k += NT) acc[k] = 0.0;
+  block_max_write<NT>(pmax, red);
   __syncthreads();
+  // |contribution| <= vol |grad N|^2 * 3 * (|c00| + 2|c01| + |c33|) * 8 max|p| for weights
+  // <= 1, plus FP32 rounding (the factor 2)
+  const double S = fixed_scale_54(48.0 * (fabs(A.m.c00) + 2.0 * fabs(A.m.c01) + fabs(A.m.c33)) *
+                                  (double)A.gbound[c] * block_max_read<NT>(red));
 
   const float c00 = (float)A.m.c00, c01 = (float)A.m.c01, c33 = (float)A.m.c33;
   const float rmin = (float)A.m.rmin, pp = (float)A.m.p;
@@ -129,21 +139,20 @@ __global__ __launch_bounds__(NT) void k_mf_apply(MfArgs A) {
     for (int a = 0; a < 4; ++a) {
       if (ln[a] < nr) {
         double* dst = &acc[3 * ln[a]];
-        __hip_atomic_fetch_add(dst + 0, (double)(s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(dst + 1, (double)(s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(dst + 2, (double)(s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        lds_add_fixed64(dst + 0, (double)(s0 * g[a][0] + s5 * g[a][1] + s4 * g[a][2]), S);
+        lds_add_fixed64(dst + 1, (double)(s5 * g[a][0] + s1 * g[a][1] + s3 * g[a][2]), S);
+        lds_add_fixed64(dst + 2, (double)(s4 * g[a][0] + s3 * g[a][1] + s2 * g[a][2]), S);
       }
     }
   }
   __syncthreads();
   double dot = 0.0;
+  const double inv = 1.0 / S;  // a power of two: exact (NaN for a non-finite p)
   for (int k = threadIdx.x; k < 3 * nr; k += NT) {
     const size_t gd = 3 * (size_t)r0 + k;
     const double pv = pown[k];
-    const double yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? pv : acc[k];
+    const double av = S == S ? fixed_value(acc[k], inv) : NAN;
+    const double yv = (A.bcmask[r0 + k / 3] >> (k % 3)) & 1u ? pv : av;
     A.y[gd] = yv;
     dot += pv * yv;
   }
@@ -161,6 +170,8 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
   const int n0 = A.node_offs[c];
   for (int k = threadIdx.x; k < 9 * nr; k += NT) D[k] = 0.0;
   __syncthreads();
+  // fixed-point sums (order-independent): |contribution| <= vol |grad N|^2 (|c01| + 2|c33|)
+  const double S = fixed_scale_54((double)A.gbound[c] * (fabs(A.m.c01) + 2.0 * fabs(A.m.c33)));
   const int l0 = A.list_offs[c], l1 = A.list_offs[c + 1];
   for (int k = l0 + threadIdx.x; k < l1; k += NT) {
     const uint2 lw = reinterpret_cast<const uint2*>(A.ent_ln)[k];
@@ -187,15 +198,16 @@ __global__ __launch_bounds__(NT) void k_mf_diag(DiagArgs A) {
         for (int j = 0; j < 3; ++j) {
           double v = (A.m.c01 + A.m.c33) * g[a][i] * g[a][j];
           if (i == j) v += A.m.c33 * gg;
-          __hip_atomic_fetch_add(dst + 3 * i + j, vw * v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          lds_add_fixed64(dst + 3 * i + j, vw * v, S);
         }
     }
   }
   __syncthreads();
+  const double inv = 1.0 / S;
   for (int lr = threadIdx.x; lr < nr; lr += NT) {
     double a[9];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) a[q] = D[9 * lr + q];
+    for (int q = 0; q < 9; ++q) a[q] = fixed_value(D[9 * lr + q], inv);
     block_inv3_masked(a, A.bcmask[r0 + lr], A.dinv + 9 * (size_t)(r0 + lr));
   }
 }

@@ -30,6 +30,7 @@ struct MfArgs {
   double* y;         // y = K p_cur through the FP32 operator (constrained rows: y = p_cur)
   double* partials;  // per-chunk partial of p_cur . y
   Material m;
+  const float* gbound;  // per chunk: bound of vol |grad N|^2 (scale of the fixed-point sums)
 };
 hipError_t launch_mf_apply(const MfArgs& a, hipStream_t s);
 
@@ -46,6 +47,7 @@ struct DiagArgs {
   const uint8_t* bcmask;
   double* dinv;
   Material m;
+  const float* gbound;  // per chunk: bound of vol |grad N|^2 (scale of the fixed-point sums)
 };
 hipError_t launch_mf_diag_inv(const DiagArgs& a, hipStream_t s);
 

@@ -322,6 +322,40 @@ def test_gpu_general_jacobian_bit_identical_rerun(mesh, monkeypatch):
     close(out[0][0], o.jacobian(u, z, rm, ci), RTOL, "K")
 
 
+def test_gpu_general_mixed_solve_bit_identical(monkeypatch):
+    """On a general mesh (box declared but the chunk kernels forced: PA_BOX_ASSEMBLY=0,
+    PA_BOX_MF=0, PA_BOX_RES=0) the FP32 operator, its block diagonal and the FP64
+    residual sum in fixed point, so a repeated mixed-precision solve takes the same
+    iterations to the same bits; the residual kernel alone matches the oracle."""
+    print("case: general-mesh mixed solve rerun", flush=True)
+    for k in ("PA_BOX_ASSEMBLY", "PA_BOX_MF", "PA_BOX_RES"):
+        monkeypatch.setenv(k, "0")
+    w = wl.make("cantilever_128x64x64", tiny=(20, 10, 10))
+    m = wl.model(w)
+    assert not m.box_assembly
+    dz = dev(w.z)
+    runs = []
+    for _ in range(2):
+        pb = Problem(m, tolerance=1e-10, check_every=8, precision="mixed")
+        u = host(pb.solution(dz))
+        runs.append((u, pb.stats.iterations, pb.stats.outer_iterations))
+        assert pb.stats.converged and pb.stats.rel_residual <= 1e-10
+    print(f"  iterations {runs[0][1]} and {runs[1][1]}", flush=True)
+    assert runs[0][1] == runs[1][1] and runs[0][2] == runs[1][2]
+    assert np.array_equal(runs[0][0], runs[1][0])
+    u64 = host(Problem(m, tolerance=1e-12).solution(dz))
+    close(runs[0][0], u64, 1e-7, "mixed vs FP64 solution")
+    # the fixed-point residual against the oracle, and twice the same bits
+    rng = np.random.default_rng(5)
+    u = rng.normal(size=m.num_dofs) * 1e-2
+    vf = VectorFunction(m)
+    r1, r2 = host(vf.value(dev(u), dz)), host(vf.value(dev(u), dz))
+    assert np.array_equal(r1, r2)
+    o = orc.Model(w.mesh.coords, w.mesh.conn, w.cfg.E, w.cfg.nu, w.cfg.p, w.cfg.rho_min,
+                  m.tractions[0].face_verts, w.traction)
+    close(r1, o.residual(u, w.z), RTOL, "residual")
+
+
 # ------------------------------------------------------------------ mixed precision
 def test_gpu_mixed_precision_solve_cfg0():
     """FP32 matrix-free PCG with FP64 residual replacement reaches the FP64 tolerance on
