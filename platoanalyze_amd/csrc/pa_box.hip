@@ -882,4 +882,104 @@ __global__ __launch_bounds__(256, 6) void k_boxmf_apply(BoxMfArgs A) {
   if (threadIdx.x == 0) A.partials[blockIdx.x] = t;
 }
 
+// k_boxmf_apply_kuhn (pa_box.hpp).  The row's p at the 15 lattice offsets the Kuhn tets
+// reach (all components in {0, 1} or all in {0, -1}) is loaded first, the weight of tet q
+// of a hex from slot q of its record; each tet's force is formed in FP32 from FP32 p and
+// weights, summed in FP32 over the four tets of each path and in FP64 over the paths, in
+// a fixed order.
+__global__ __launch_bounds__(256) void k_boxmf_apply_kuhn(BoxMfArgs A) {
+  __shared__ double sh[4];
+  const int px = A.nx + 2, pxy = (A.nx + 2) * (A.ny + 2);
+  const float ih[3] = {A.ih[0], A.ih[1], A.ih[2]};
+  const float lam = A.lam, mu = A.mu, vol = A.vol;
+  const float4* __restrict__ pl = A.plat;
+  const float* __restrict__ W = A.W;
+  const XcdSpan sp = xcd_span(A.nrows);
+  double dot = 0.0;
+  for (int row = sp.lo + sp.j * 256 + (int)threadIdx.x; row < sp.hi; row += sp.bx * 256) {
+    const int g = A.gpack[row];
+    const int hr = ((g & 1023) + 1) + px * ((((g >> 10) & 1023) + 1) + (A.ny + 2) * ((g >> 20) + 1));
+    auto off = [&](int dx, int dy, int dz) { return dx + px * dy + pxy * dz; };
+    float3 P[3][3][3];  // p at offset (dx, dy, dz) in P[dx + 1][dy + 1][dz + 1]
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const bool reach = (dx >= 0 && dy >= 0 && dz >= 0) || (dx <= 0 && dy <= 0 && dz <= 0);
+          if (reach) {
+            const float4 v = pl[hr + off(dx, dy, dz)];
+            P[dx + 1][dy + 1][dz + 1] = make_float3(v.x, v.y, v.z);
+          }
+        }
+    double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      float yq[3] = {0.0f, 0.0f, 0.0f};  // the four tets of path q, then added in FP64
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {  // the row is local vertex m of tet q of its hex
+        // offsets of the tet's four vertices relative to the row point
+        int d[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) d[v][k] = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            if (l < v) d[v][kKuhnPath[q][l]] += 1;
+            if (l < m) d[v][kKuhnPath[q][l]] -= 1;
+          }
+        const float w = W[6 * (hr + off(d[0][0], d[0][1], d[0][2])) + q];
+        float G[3][3];  // G[i][j] = d p_i / d x_j
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          const int c = kKuhnPath[q][l];
+          const float3 a = P[d[l][0] + 1][d[l][1] + 1][d[l][2] + 1];
+          const float3 b = P[d[l + 1][0] + 1][d[l + 1][1] + 1][d[l + 1][2] + 1];
+          G[0][c] = (b.x - a.x) * ih[c];
+          G[1][c] = (b.y - a.y) * ih[c];
+          G[2][c] = (b.z - a.z) * ih[c];
+        }
+        const float tr = G[0][0] + G[1][1] + G[2][2];
+        float S[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) S[i][i] = fmaf(2.0f * mu, G[i][i], lam * tr);
+        S[0][1] = S[1][0] = mu * (G[0][1] + G[1][0]);
+        S[0][2] = S[2][0] = mu * (G[0][2] + G[2][0]);
+        S[1][2] = S[2][1] = mu * (G[1][2] + G[2][1]);
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if (m > 0) {
+          const int c = kKuhnPath[q][m - 1];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) f[i] = S[i][c] * ih[c];
+        }
+        if (m < 3) {
+          const int c = kKuhnPath[q][m];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) f[i] = fmaf(-S[i][c], ih[c], f[i]);
+        }
+        const float vw = vol * w;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) yq[i] = fmaf(vw, f[i], yq[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) y[i] += (double)yq[i];
+    }
+    const uint32_t bm = A.bcmask[row];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const size_t dd = 3 * (size_t)row + i;
+      const double pc = A.p_cur[dd];
+      const double yv = (bm >> i) & 1u ? pc : y[i];
+      A.y[dd] = yv;
+      dot += pc * yv;
+    }
+  }
+  const double t = block_reduce_all<256>(dot, sh);
+  if (threadIdx.x == 0) A.partials[blockIdx.x] = t;
+}
+
 }  // namespace pa

@@ -99,6 +99,7 @@ struct BoxMfArgs {
   const int* gpack;        // per node: i | j << 10 | k << 20
   const BoxStencil* S;
   const float* W;          // [padded hexes][6] FP32 element weights, zero on the ghost layer
+                           // (mesh tet order; Kuhn-path order for k_boxmf_apply_kuhn)
   const float* B;          // [16][6][9] FP32 unit-weight blocks
   const uint8_t* bcmask;   // bit i: dof 3n+i constrained
   const double* z;         // FP64 vectors in node order
@@ -111,7 +112,15 @@ struct BoxMfArgs {
   double* partials;        // one p.y partial per work-group of k_boxmf_apply
   int nrows;
   int nx, ny, nz;          // hexes per axis
+  // k_boxmf_apply_kuhn (W in Kuhn-path order): 1 / spacing per axis, tet volume, Lame
+  // constants
+  float ih[3];
+  float vol, lam, mu;
 };
+
+// The six tets of a hex split along its main diagonal: tet q visits corners 0,
+// e_a, e_a + e_b, e_a + e_b + e_c for the axis path (a, b, c) = kKuhnPath[q].
+constexpr int kKuhnPath[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
 // R = K u - f (f subtracted when sub_load) without forming K, in FP64, from the element
 // weights A.W and u at the lattice points (A.ulat); the rows sum in a fixed order.
@@ -128,5 +137,14 @@ __global__ void k_boxmf_scatter(BoxMfArgs A);
 // y = K p_cur and the work-group's p.y partial; 8 lanes per row, 32-row tiles, a
 // persistent XCD-contiguous grid like k_box_jac.
 __global__ void k_boxmf_apply(BoxMfArgs A);
+
+// The same product for lattices split into the six main-diagonal Kuhn tets, one row per
+// lane from the 24 tets around its lattice point: in tet (a, b, c) column a of the
+// displacement gradient is (p(e_a) - p(0)) / h_a, column b (p(e_a + e_b) - p(e_a)) / h_b,
+// column c the last edge over h_c; the force on local vertex m is
+// vol w sigma g_m with g_m = e_{path[m-1]} / h - e_{path[m]} / h (terms present for
+// 0 < m and m < 3).  No block tables; the lattice p must be zero on the ghost layer.
+// Persistent XCD-contiguous grid over the rows, one p.y partial per work-group.
+__global__ void k_boxmf_apply_kuhn(BoxMfArgs A);
 
 }  // namespace pa

@@ -204,6 +204,12 @@ struct pa_problem {
   DevArray<double> box_mf_part;
   int box_grid_mf = 1024;
   bool box_mf = true;  // PA_BOX_MF=0 keeps the chunk-based FP32 apply on lattice meshes
+  // k_boxmf_apply_kuhn in place of k_boxmf_apply: hexes split into the six main-diagonal
+  // Kuhn tets (PA_BOX_MF_KUHN=0 keeps k_boxmf_apply); its FP32 weights are stored in
+  // Kuhn-path order (box_corners_kuhn: the corner sets in kKuhnPath order)
+  bool box_kuhn = false;
+  DevArray<uint8_t> box_corners_kuhn;
+  double box_step[3] = {0.0, 0.0, 0.0};
   // k_box_jac_ws (store and compute waves specialised; PA_BOX_WS=0 selects k_box_jac)
   bool box_ws = true;
   int box_seg_cap = 0, box_grid_ws = 512;
@@ -993,11 +999,20 @@ static int mixed_setup(pa_handle h, const double* z) {
     // diagonal from the FP64 weights, both summed in a fixed order
     if (h->box_W32.n != h->box_W.n) {
       PA_HIP(h->box_W32.alloc(h->box_W.n));
-      PA_HIP(h->box_plat.alloc(h->box_zlat.n));
+      // FP32 p at the lattice points, zero on the ghost layer, which k_boxmf_scatter never
+      // writes, and on a tail of one padded plane and row past the end (k_boxmf_apply_kuhn
+      // reads one point beyond the last plane; the tets there have zero weight)
+      const size_t px = (size_t)h->box.nx + 1, pxy = px * ((size_t)h->box.ny + 1);
+      PA_HIP(h->box_plat.alloc(h->box_zlat.n + pxy + px + 1));
+      pa::k_fill<<<grid_for(2 * h->box_plat.n), 256, 0, s>>>(reinterpret_cast<double*>(h->box_plat.p),
+                                                            2 * h->box_plat.n, 0.0);
+      PA_HIP(hipGetLastError());
       PA_HIP(h->box_mf_part.alloc(pa::kRedBlocks));
     }
     const pa::BoxArgs a = box_args(h, nullptr, z, nullptr, nullptr);
-    box_weights(h, a, h->box_W32.p);
+    pa::BoxArgs a32 = a;  // FP32 weights for the operator, in Kuhn-path order for k_boxmf_apply_kuhn
+    if (h->box_kuhn) a32.corners = h->box_corners_kuhn.p;
+    box_weights(h, a32, h->box_W32.p);
     box_weights(h, a, a.W);
     pa::k_box_diag_inv<<<grid_for(((size_t)nn + 31) / 32, 1, 16384), 256, 0, s>>>(a, h->bcmask.p, h->m_dinv.p);
     PA_HIP(hipGetLastError());
@@ -1033,9 +1048,19 @@ static int mf_apply(pa_handle h, const pa::MfArgs& ma, const double** P, int* np
     b.ny = h->box.ny - 1;
     b.nz = h->box.nz - 1;
     const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
-    const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_mf);
+    int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_mf);
     pa::k_boxmf_scatter<<<grid_for((size_t)h->nn, 256, 16384), 256, 0, s>>>(b);
-    pa::k_boxmf_apply<<<G, 256, 0, s>>>(b);
+    if (h->box_kuhn) {
+      const pa::Material m = h->material();
+      for (int k = 0; k < 3; ++k) b.ih[k] = (float)(1.0 / h->box_step[k]);
+      b.vol = (float)(h->box_step[0] * h->box_step[1] * h->box_step[2] / 6.0);
+      b.lam = (float)m.c01;
+      b.mu = (float)m.c33;
+      G = std::min(pa::kRedBlocks, ((h->nn + 255) / 256 + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd);
+      pa::k_boxmf_apply_kuhn<<<G, 256, 0, s>>>(b);
+    } else {
+      pa::k_boxmf_apply<<<G, 256, 0, s>>>(b);
+    }
     PA_HIP(hipGetLastError());
     *P = h->box_mf_part.p;
     *np = G;
@@ -1471,6 +1496,22 @@ static int box_fast_setup(pa_handle h) {
     const size_t lattice_bytes = (size_t)(6 + 3) * sizeof(double) * (hx + 2) * (hy + 2) * (hz + 2);
     h->box_ahead = lattice_bytes > kBoxAheadBytes;
     if (const char* e = std::getenv("PA_BOX_AHEAD")) h->box_ahead = std::atoi(e) != 0;
+  }
+  {
+    // the six tets of every hex are the main-diagonal Kuhn paths (sorted corner sets
+    // 0, e_a, e_a + e_b, 7)
+    h->box_kuhn = true;
+    std::vector<uint8_t> ck(6 * 4);
+    for (int q = 0; q < 6 && h->box_kuhn; ++q) {
+      const int a = pa::kKuhnPath[q][0], b = pa::kKuhnPath[q][1];
+      const std::array<uint8_t, 4> want = {0, (uint8_t)(1 << a), (uint8_t)((1 << a) | (1 << b)), 7};
+      if (std::find(types.begin(), types.end(), want) == types.end()) h->box_kuhn = false;
+      for (int v = 0; v < 4; ++v) ck[4 * q + v] = want[v];
+    }
+    if (const char* e = std::getenv("PA_BOX_MF_KUHN"))
+      if (std::atoi(e) == 0) h->box_kuhn = false;
+    if (h->box_kuhn) PA_HIP(h->box_corners_kuhn.upload(ck));
+    for (int k = 0; k < 3; ++k) h->box_step[k] = step[k];
   }
   h->box_fast = true;
   return PA_OK;
