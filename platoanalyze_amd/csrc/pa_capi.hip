@@ -107,6 +107,7 @@ struct MgLevel {
   DevArray<int32_t> grid;          // (i, j, k) per node
   DevArray<int32_t> node_of_grid;  // node per grid point
   DevArray<double> r, x, y, d, t, rr, w, vals, dinv;  // w: nodal stiffness weight
+  DevArray<float> vals32;          // FP32 copy of vals for the mixed solve's smoothing
   DevArray<double> pw;             // smoothing and spectral-estimate vector
   double inv_theta = 0.0, c1 = 0.0, c2 = 0.0, lam = 0.0, lmax = 0.0;
   ~MgLevel();
@@ -183,6 +184,7 @@ struct pa_problem {
   // (degree 1 there: 74-76 iterations); interval [lambda / 60, 1.1 lambda].
   bool mg_fp32_smooth = true;
   int mg_pre0 = 1, mg_post0 = 1, mg_degc = 2;
+  bool mg_f32 = true;  // mixed solve: coarse smoothing on FP32 matrix copies (PA_MG_F32=0: FP64)
   double mg_ratio = 60.0;
   DevArray<double> g_r, g_z, g_p, g_q, g_pc;  // PCG vectors of the multigrid solve
   // lattice assembly (pa_box.hip): enabled by box_fast_setup when the box's hexes all
@@ -1636,7 +1638,8 @@ static int mg_build(pa_handle h) {
   if (!h->mg.empty()) return PA_OK;
   if (!h->has_box) return fail(PA_ERR_STATE, "multigrid needs the box grid (pa_set_box)");
   // tuning knobs: PA_MG_FP32_SMOOTH=0|1, PA_MG_DEG0=<pre><post> (each 1 or 2) on level 0,
-  // PA_MG_DEGC=1|2 on the coarser levels, PA_MG_RATIO = lambda_max / lambda_min
+  // PA_MG_DEGC=1|2 on the coarser levels, PA_MG_RATIO = lambda_max / lambda_min,
+  // PA_MG_F32=0|1 (FP32 coarse matrices in the mixed solve's smoothing)
   if (const char* e = std::getenv("PA_MG_FP32_SMOOTH")) h->mg_fp32_smooth = std::atoi(e) != 0;
   if (const char* e = std::getenv("PA_MG_DEG0")) {
     if (std::strlen(e) == 2 && (e[0] == '1' || e[0] == '2') && (e[1] == '1' || e[1] == '2')) {
@@ -1645,6 +1648,7 @@ static int mg_build(pa_handle h) {
     }
   }
   if (const char* e = std::getenv("PA_MG_DEGC")) h->mg_degc = std::atoi(e) == 1 ? 1 : 2;
+  if (const char* e = std::getenv("PA_MG_F32")) h->mg_f32 = std::atoi(e) != 0;
   if (const char* e = std::getenv("PA_MG_RATIO")) h->mg_ratio = std::max(1.5, std::atof(e));
   auto L0 = std::make_unique<MgLevel>();
   L0->h = h;
@@ -1755,12 +1759,14 @@ static const double* mg_dinv(pa_handle h, size_t l, bool mixed) {
   return mixed ? h->m_dinv.p : h->w_dinv.p;
 }
 
-// Fused block-CRS smoothing step on level l (l > 0, or level 0 of the FP64 solve).
+// Fused block-CRS smoothing step on level l (l > 0, or level 0 of the FP64 solve); the
+// mixed solve's coarse levels read their FP32 copy of the matrix.
 static int mg_rows(pa_handle h, size_t l, int mode, const double* vals0, const double* x, const double* r,
-                   double* d, double* out, double* y, const double* base) {
+                   double* d, double* out, double* y, const double* base, bool mixed) {
   MgLevel& L = *h->mg[l];
   pa::MgRowArgs a{};
   a.vals = l == 0 ? vals0 : L.vals.p;
+  a.vals32 = l > 0 && mixed && h->mg_f32 ? L.vals32.p : nullptr;
   a.rowmap = L.h->rowmap.p;
   a.col = L.h->colidx.p;
   a.x = x;
@@ -1803,9 +1809,9 @@ static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const do
   if (fused) {
     if (pre == 2) {
       x1 = L.pw.p;
-      if (int rc = mg_rows(h, l, pa::kMgCheb, vals0, x, r, L.d.p, x1, nullptr, nullptr)) return rc;
+      if (int rc = mg_rows(h, l, pa::kMgCheb, vals0, x, r, L.d.p, x1, nullptr, nullptr, mixed)) return rc;
     }
-    if (int rc = mg_rows(h, l, pa::kMgRes, vals0, x1, r, nullptr, L.rr.p, nullptr, nullptr)) return rc;
+    if (int rc = mg_rows(h, l, pa::kMgRes, vals0, x1, r, nullptr, L.rr.p, nullptr, nullptr, mixed)) return rc;
   } else {
     if (pre == 2) {
       if (int rc = mg_apply(h, l, x, L.t.p, vals0, mixed)) return rc;
@@ -1822,9 +1828,9 @@ static int mg_vcycle(pa_handle h, size_t l, const double* r, double* x, const do
   PA_HIP(pa::launch_mg_prolong_add(C.x.p, x1, L.grid.p, C.node_of_grid.p, C.dims, L.h->bcmask.p, L.nn, s));
   // post-smoothing on the remaining residual
   if (fused) {
-    if (int rc = mg_rows(h, l, pa::kMgResCheb0, vals0, x1, r, L.d.p, L.rr.p, L.y.p, nullptr)) return rc;
+    if (int rc = mg_rows(h, l, pa::kMgResCheb0, vals0, x1, r, L.d.p, L.rr.p, L.y.p, nullptr, mixed)) return rc;
     if (post == 2) {
-      if (int rc = mg_rows(h, l, pa::kMgChebAdd, vals0, L.y.p, L.rr.p, L.d.p, x, nullptr, x1)) return rc;
+      if (int rc = mg_rows(h, l, pa::kMgChebAdd, vals0, L.y.p, L.rr.p, L.d.p, x, nullptr, x1, mixed)) return rc;
     } else {
       PA_HIP(pa::launch_add2(x, x1, L.y.p, nd, s));
     }
@@ -2008,6 +2014,10 @@ static int mg_setup(pa_handle h, const double* z, const double* vals0, bool mixe
     if (int rc = pa_apply_constraints(L.h, L.vals.p, L.rr.p, 0.0)) return rc;
     pa::k_diag_inv<<<grid_for(L.nn), 256, 0, s>>>(L.vals.p, L.h->diag_slot.p, L.dinv.p, L.nn);
     PA_HIP(hipGetLastError());
+    if (mixed && h->mg_f32 && l + 1 < nl) {  // smoothed levels: the FP32 copy they stream
+      if (L.vals32.n != L.vals.n) PA_HIP(L.vals32.alloc(L.vals.n));
+      PA_HIP(pa::launch_to_f32(L.vals.p, L.vals32.p, L.vals.n, s));
+    }
   }
   // Chebyshev interval [lmax / ratio, lmax], lmax = 1.1 x the largest Ritz value of D^-1 K
   // per smoothed level

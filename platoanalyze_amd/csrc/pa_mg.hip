@@ -260,18 +260,25 @@ __device__ __forceinline__ void mg_row_epilogue(const MgRowArgs& A, int row, con
   }
 }
 
-template <int MODE>
+template <int MODE, typename V>
 __global__ __launch_bounds__(kT) void k_mg_rows(MgRowArgs A) {
   constexpr int kRows = kT / kLanesPerRow;
   const int lane = threadIdx.x % kLanesPerRow;
   const XcdSpan sp = xcd_span(A.nrows);
   const double* __restrict__ x = A.x;
+  const V* __restrict__ vals;
+  if constexpr (sizeof(V) == 4) vals = A.vals32;
+  else vals = A.vals;
   auto X = [&](int c, int j) { return x[3 * (size_t)c + j]; };
   for (int row = sp.lo + sp.j * kRows + threadIdx.x / kLanesPerRow; row < sp.hi; row += sp.bx * kRows) {
     double acc[3];
-    spmv_row(A.vals, A.rowmap, A.col, row, lane, X, acc);
+    spmv_row(vals, A.rowmap, A.col, row, lane, X, acc);
     if (lane == 0) mg_row_epilogue<MODE>(A, row, acc);
   }
+}
+
+__global__ __launch_bounds__(kT) void k_to_f32(const double* __restrict__ a, float* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)kT + threadIdx.x; i < n; i += (size_t)gridDim.x * kT) b[i] = (float)a[i];
 }
 
 __global__ __launch_bounds__(kT) void k_scale_rsqrt(double* __restrict__ y, size_t n, const double* __restrict__ scal,
@@ -291,13 +298,27 @@ hipError_t launch_mg_restrict(const double* rf, double* rc, const int32_t* cgrid
 }
 hipError_t launch_mg_rows(int mode, const MgRowArgs& a, hipStream_t s) {
   const int grid = red_grid((size_t)a.nrows, kT / kLanesPerRow);
-  switch (mode) {
-    case kMgRes: k_mg_rows<kMgRes><<<grid, kT, 0, s>>>(a); break;
-    case kMgResCheb0: k_mg_rows<kMgResCheb0><<<grid, kT, 0, s>>>(a); break;
-    case kMgCheb: k_mg_rows<kMgCheb><<<grid, kT, 0, s>>>(a); break;
-    case kMgChebAdd: k_mg_rows<kMgChebAdd><<<grid, kT, 0, s>>>(a); break;
-    default: return hipErrorInvalidValue;
+  if (a.vals32) {
+    switch (mode) {
+      case kMgRes: k_mg_rows<kMgRes, float><<<grid, kT, 0, s>>>(a); break;
+      case kMgResCheb0: k_mg_rows<kMgResCheb0, float><<<grid, kT, 0, s>>>(a); break;
+      case kMgCheb: k_mg_rows<kMgCheb, float><<<grid, kT, 0, s>>>(a); break;
+      case kMgChebAdd: k_mg_rows<kMgChebAdd, float><<<grid, kT, 0, s>>>(a); break;
+      default: return hipErrorInvalidValue;
+    }
+  } else {
+    switch (mode) {
+      case kMgRes: k_mg_rows<kMgRes, double><<<grid, kT, 0, s>>>(a); break;
+      case kMgResCheb0: k_mg_rows<kMgResCheb0, double><<<grid, kT, 0, s>>>(a); break;
+      case kMgCheb: k_mg_rows<kMgCheb, double><<<grid, kT, 0, s>>>(a); break;
+      case kMgChebAdd: k_mg_rows<kMgChebAdd, double><<<grid, kT, 0, s>>>(a); break;
+      default: return hipErrorInvalidValue;
+    }
   }
+  return hipGetLastError();
+}
+hipError_t launch_to_f32(const double* a, float* b, size_t n, hipStream_t s) {
+  k_to_f32<<<blocks_of(n), kT, 0, s>>>(a, b, n);
   return hipGetLastError();
 }
 hipError_t launch_scale_rsqrt(double* y, size_t n, const double* scal, int slot, hipStream_t s) {

@@ -56,6 +56,7 @@ hipError_t launch_dense_gemv(const double* A, const double* x, double* y, int n,
 enum MgRowMode { kMgRes = 0, kMgResCheb0 = 1, kMgCheb = 2, kMgChebAdd = 3 };
 struct MgRowArgs {
   const double* vals;
+  const float* vals32;  // when set: an FP32 copy of the level's matrix, read in place of vals
   const int32_t* rowmap;
   const int32_t* col;
   const double* x;
@@ -69,6 +70,8 @@ struct MgRowArgs {
   int nrows;
 };
 hipError_t launch_mg_rows(int mode, const MgRowArgs& a, hipStream_t s);
+// b = (float) a, n values
+hipError_t launch_to_f32(const double* a, float* b, size_t n, hipStream_t s);
 // y *= 1 / sqrt(scal[slot])  (power-iteration normalisation without a host round trip)
 hipError_t launch_scale_rsqrt(double* y, size_t n, const double* scal, int slot, hipStream_t s);
 

@@ -191,3 +191,27 @@ def test_gpu_mg_high_contrast_oc_design(precision):
     assert st.converged and st.rel_residual <= 1e-9
     assert rel_err(u, u64) <= 1e-6
     assert st.iterations <= 100  # 62 measured in both precisions
+
+
+def test_gpu_mg_mixed_fp32_coarse_smoothing(monkeypatch):
+    """The mixed solve smooths its coarse levels with FP32 copies of their matrices
+    (default) or the FP64 matrices (PA_MG_F32=0): both reach the FP64 Jacobi solution, the
+    FP32 copies cost at most two more iterations, and a repeated solve is bit-identical."""
+    print("case: mixed multigrid, FP32 and FP64 coarse smoothing", flush=True)
+    w = wl.make("cantilever_128x64x64", tiny=(32, 16, 16))
+    dz = dev(w.z)
+    runs = {}
+    for f32 in ("1", "0"):
+        monkeypatch.setenv("PA_MG_F32", f32)
+        m = wl.model(w)
+        if f32 == "1":
+            u64 = host(Problem(m, tolerance=1e-12).solution(dz))
+        pb = Problem(m, tolerance=1e-10, precision="mixed", preconditioner="multigrid")
+        u = host(pb.solution(dz))
+        assert pb.stats.converged and pb.stats.rel_residual <= 1e-10
+        assert rel_err(u, u64) <= 1e-7
+        runs[f32] = (u, pb.stats.iterations)
+        if f32 == "1":
+            assert np.array_equal(host(pb.solution(dz)), u)
+    print(f"  iterations FP32 coarse {runs['1'][1]}, FP64 coarse {runs['0'][1]}", flush=True)
+    assert runs["1"][1] <= runs["0"][1] + 2
