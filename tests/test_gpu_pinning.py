@@ -241,3 +241,27 @@ def test_gpu_linspace_axis_keeps_lattice_assembly():
     u, z = dev(rng.normal(size=3 * lin.num_nodes)), dev(rng.uniform(0.1, 1.0, lin.num_nodes))
     close(host(VectorFunction(mb).gradient_u(u, z).entries), host(VectorFunction(mg).gradient_u(u, z).entries),
           RTOL, "lattice vs general")
+
+
+@pytest.mark.parametrize("name,p,rmin,L", [("RAMP", 3.0, 1e-3, 1.0), ("Heaviside", 2.0, 1e-3, 1.2),
+                                           ("NoPenalty", 1.0, 0.0, 1.0)])
+def test_gpu_lattice_fp32_operator_penalty_models(name, p, rmin, L):
+    """The mixed solves' lattice FP32 operator takes the penalised element weights of the
+    handle's model: under RAMP, Heaviside and no penalty, on a varying density, mixed
+    Jacobi and mixed multigrid reach the FP64 Jacobi solution."""
+    print(f"case: lattice FP32 operator, penalty {name}", flush=True)
+    mesh = build_box(16, 8, 8, 2.0, 1.0, 1.0)
+    fv, fe = faces_on_plane(mesh, 0, 2.0)
+    fixed = np.flatnonzero(mesh.coords[:, 0] == 0.0).astype(np.int32)
+    bcs = [EssentialBC(fixed, d, 0.0) for d in range(3)]
+    tr = [Traction(fv, fe, (0.0, 0.0, -1.0))]
+    z = dev(np.random.default_rng(8).uniform(0.05, 1.0, mesh.num_nodes))
+    m = ElasticModel(mesh, 1.0, 0.3, p, rmin, bcs, tr, box=(16, 8, 8), penalty_model=name,
+                     regularization_length=L)
+    u64 = host(Problem(m, tolerance=1e-12).solution(z))
+    for pre in ("jacobi", "multigrid"):
+        pb = Problem(m, tolerance=1e-10, precision="mixed", preconditioner=pre)
+        u = host(pb.solution(z))
+        err = np.abs(u - u64).max() / np.abs(u64).max()
+        print(f"  mixed {pre}: {pb.stats.iterations} it, err {err:.2e}", flush=True)
+        assert pb.stats.converged and err <= 1e-7
