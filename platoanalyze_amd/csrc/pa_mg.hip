@@ -36,24 +36,24 @@ __global__ __launch_bounds__(kT) void k_mg_restrict(const double* __restrict__ r
                                                     double* __restrict__ cx, double* __restrict__ cd) {
   for (int c = blockIdx.x * kT + threadIdx.x; c < ncn; c += gridDim.x * kT) {
     const int I = cgrid[3 * c], J = cgrid[3 * c + 1], K = cgrid[3 * c + 2];
+    // branch-free over the 27 fine points: one outside the fine grid reads the coarse
+    // node's own fine point with weight 0 (adds an exact zero), so all lookups issue at once
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int a = -1; a <= 1; ++a) {
-      const int i = 2 * I + a;
-      if (i < 0 || i >= fd.nx) continue;
-      for (int b = -1; b <= 1; ++b) {
-        const int j = 2 * J + b;
-        if (j < 0 || j >= fd.ny) continue;
+#pragma unroll
+    for (int a = -1; a <= 1; ++a)
+#pragma unroll
+      for (int b = -1; b <= 1; ++b)
+#pragma unroll
         for (int e = -1; e <= 1; ++e) {
-          const int k = 2 * K + e;
-          if (k < 0 || k >= fd.nz) continue;
-          const double w = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (e ? 0.5 : 1.0);
-          const size_t f = 3 * (size_t)fnode[((size_t)i * fd.ny + j) * fd.nz + k];
+          const int i = 2 * I + a, j = 2 * J + b, k = 2 * K + e;
+          const bool in = (unsigned)i < (unsigned)fd.nx && (unsigned)j < (unsigned)fd.ny && (unsigned)k < (unsigned)fd.nz;
+          const double w = in ? (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) * (e ? 0.5 : 1.0) : 0.0;
+          const size_t f = 3 * (size_t)fnode[((size_t)(in ? i : 2 * I) * fd.ny + (in ? j : 2 * J)) * fd.nz +
+                                             (in ? k : 2 * K)];
           s0 += w * rf[f];
           s1 += w * rf[f + 1];
           s2 += w * rf[f + 2];
         }
-      }
-    }
     const uint32_t bm = cbc[c];
     const double v[3] = {(bm & 1u) ? 0.0 : s0, (bm & 2u) ? 0.0 : s1, (bm & 4u) ? 0.0 : s2};
     const size_t o = 3 * (size_t)c;
@@ -81,12 +81,19 @@ __global__ __launch_bounds__(kT) void k_mg_prolong_add(const double* __restrict_
     const int I0 = i >> 1, J0 = j >> 1, K0 = k >> 1;
     const int nI = (i & 1) ? 2 : 1, nJ = (j & 1) ? 2 : 1, nK = (k & 1) ? 2 : 1;
     const double wI = (i & 1) ? 0.5 : 1.0, wJ = (j & 1) ? 0.5 : 1.0, wK = (k & 1) ? 0.5 : 1.0;
+    // branch-free over the 8 candidates: an absent one reads coarse point (I0, J0, K0) with
+    // weight 0 (adds an exact zero)
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int a = 0; a < nI; ++a)
-      for (int b = 0; b < nJ; ++b)
-        for (int e = 0; e < nK; ++e) {
-          const size_t c = 3 * (size_t)cnode[((size_t)(I0 + a) * cd.ny + (J0 + b)) * cd.nz + (K0 + e)];
-          const double w = wI * wJ * wK;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const bool in = a < nI && b < nJ && e < nK;
+          const size_t c = 3 * (size_t)cnode[((size_t)(I0 + (in ? a : 0)) * cd.ny + (J0 + (in ? b : 0))) * cd.nz +
+                                             (K0 + (in ? e : 0))];
+          const double w = in ? wI * wJ * wK : 0.0;
           s0 += w * ec[c];
           s1 += w * ec[c + 1];
           s2 += w * ec[c + 2];
