@@ -783,7 +783,7 @@ __global__ __launch_bounds__(256) void k_boxmf_scatter(BoxMfArgs A) {
       const size_t d = 3 * (size_t)n + i;
       const double t = A.first ? A.z[d] : fma(beta, A.p_prev[d], A.z[d]);
       v[i] = (bm >> i) & 1u ? 0.0 : t;
-      A.p_cur[d] = v[i];
+      if (A.p_cur) A.p_cur[d] = v[i];
     }
     A.plat[hp] = make_float4((float)v[0], (float)v[1], (float)v[2], 0.0f);
   }
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(256, 6) void k_boxmf_apply(BoxMfArgs A) {
     }
     if (live && j < 3) {
       const size_t d = 3 * (size_t)row + j;
-      const double pc = A.p_cur[d];
+      const double pc = A.p_cur ? A.p_cur[d] : 0.0;
       const double yv = (A.bcmask[row] >> j) & 1u ? pc : (j == 0 ? s[0] : j == 1 ? s[1] : s[2]);
       A.y[d] = yv;
       dot += pc * yv;
@@ -969,13 +969,17 @@ __global__ __launch_bounds__(256) void k_boxmf_apply_kuhn(BoxMfArgs A) {
       for (int i = 0; i < 3; ++i) y[i] += (double)yq[i];
     }
     const uint32_t bm = A.bcmask[row];
+    // constrained rows: y = p there, which is 0 (masked); the p.y partial from the FP32 p
+    // the operator acts on (its rounding moves p.y by ~1e-7 relative, below the operator's
+    // own FP32 error), so the FP64 p is not read
+    const float3 pr = P[1][1][1];
+    const double pf[3] = {pr.x, pr.y, pr.z};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const size_t dd = 3 * (size_t)row + i;
-      const double pc = A.p_cur[dd];
-      const double yv = (bm >> i) & 1u ? pc : y[i];
+      const double yv = (bm >> i) & 1u ? 0.0 : y[i];
       A.y[dd] = yv;
-      dot += pc * yv;
+      dot += pf[i] * yv;
     }
   }
   const double t = block_reduce_all<256>(dot, sh);

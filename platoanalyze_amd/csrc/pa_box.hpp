@@ -104,7 +104,7 @@ struct BoxMfArgs {
   const uint8_t* bcmask;   // bit i: dof 3n+i constrained
   const double* z;         // FP64 vectors in node order
   const double* p_prev;
-  double* p_cur;
+  double* p_cur;           // null: no FP64 copy of p (constrained rows give y = 0)
   int first;               // p = z (else p = z + scal[kBeta] p_prev)
   const double* scal;
   float4* plat;            // [padded lattice] p (FP32) at the node's lattice point

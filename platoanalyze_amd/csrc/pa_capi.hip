@@ -1731,7 +1731,9 @@ static int mg_apply(pa_handle h, size_t l, const double* x, double* y, const dou
       ma.bcmask = h->bcmask.p;
       ma.z = x;
       ma.p_prev = nullptr;
-      ma.p_cur = h->g_pc.p;
+      // the lattice operator needs no FP64 copy of x (constrained rows give y = 0, the
+      // masked value, and the p.y partials go unused); the chunk apply writes one
+      ma.p_cur = h->box_fast && h->box_mf ? nullptr : h->g_pc.p;
       ma.first = 1;
       ma.scal = h->scal.p;
       ma.y = y;
