@@ -137,16 +137,20 @@ __global__ __launch_bounds__(kT) void k_simp_w(const double* z, double* w, int n
 
 __global__ __launch_bounds__(kT) void k_cheb_first(double* __restrict__ x, double* __restrict__ d,
                                                    const double* __restrict__ r, const double* __restrict__ dinv,
-                                                   double inv_theta, int nn) {
+                                                   double inv_theta, int nn, const double* __restrict__ t) {
   for (int n = blockIdx.x * kT + threadIdx.x; n < nn; n += gridDim.x * kT) {
     const size_t o = 3 * (size_t)n;
-    const double rv[3] = {r[o], r[o + 1], r[o + 2]};
+    double rv[3] = {r[o], r[o + 1], r[o + 2]};
+    if (t) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) rv[i] -= t[o + i];
+    }
     double zv[3];
     dinv3(dinv + 9 * (size_t)n, rv, zv);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       const double v = inv_theta * zv[i];
-      d[o + i] = v;
+      if (d) d[o + i] = v;
       x[o + i] = v;
     }
   }
@@ -347,8 +351,8 @@ hipError_t launch_simp_w(const double* z, double* w, int n, const Material& m, h
   return hipGetLastError();
 }
 hipError_t launch_cheb_first(double* x, double* d, const double* r, const double* dinv, double inv_theta, int nn,
-                             hipStream_t s) {
-  k_cheb_first<<<blocks_of(nn), kT, 0, s>>>(x, d, r, dinv, inv_theta, nn);
+                             hipStream_t s, const double* t) {
+  k_cheb_first<<<blocks_of(nn), kT, 0, s>>>(x, d, r, dinv, inv_theta, nn, t);
   return hipGetLastError();
 }
 hipError_t launch_cheb_next(double* x, double* d, const double* r, const double* Kx, const double* dinv, double c1,

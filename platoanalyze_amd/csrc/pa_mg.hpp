@@ -34,10 +34,11 @@ hipError_t launch_mg_wrestrict(const double* wf, double* wc, const int32_t* cgri
 hipError_t launch_simp_w(const double* z, double* w, int n, const Material& m, hipStream_t s);
 
 // Chebyshev smoothing of K x = r with block-Jacobi D^-1, from x = 0:
-//   first: d = D^-1 r / theta; x = d
+//   first: d = D^-1 r / theta; x = d  (r - t in place of r when t is given; d not
+//          written when null)
 //   next:  d = c1 d + c2 D^-1 (r - Kx); x += d
 hipError_t launch_cheb_first(double* x, double* d, const double* r, const double* dinv, double inv_theta, int nn,
-                             hipStream_t s);
+                             hipStream_t s, const double* t = nullptr);
 hipError_t launch_cheb_next(double* x, double* d, const double* r, const double* Kx, const double* dinv, double c1,
                             double c2, int nn, hipStream_t s);
 // out = a - b ; y += x ; y = D^-1 x
