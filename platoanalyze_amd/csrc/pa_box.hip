@@ -986,4 +986,89 @@ __global__ __launch_bounds__(256) void k_boxmf_apply_kuhn(BoxMfArgs A) {
   if (threadIdx.x == 0) A.partials[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(256) void k_box_res_kuhn(BoxArgs A, int sub_load) {
+  const int px = A.nx + 2, pxy = (A.nx + 2) * (A.ny + 2);
+  const double ih[3] = {A.ih[0], A.ih[1], A.ih[2]};
+  const double lam = A.m.c01, mu = A.m.c33, vol = A.vol;
+  int tp[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) tp[q] = A.tperm[q];
+  const double* __restrict__ ul = A.ulat;
+  const double* __restrict__ W = A.W;
+  const XcdSpan sp = xcd_span(A.nrows);
+  for (int row = sp.lo + sp.j * 256 + (int)threadIdx.x; row < sp.hi; row += sp.bx * 256) {
+    const int g = A.gpack[row];
+    const int hr = ((g & 1023) + 1) + px * ((((g >> 10) & 1023) + 1) + (A.ny + 2) * ((g >> 20) + 1));
+    auto off = [&](int dx, int dy, int dz) { return dx + px * dy + pxy * dz; };
+    double3 P[3][3][3];  // u at offset (dx, dy, dz) in P[dx + 1][dy + 1][dz + 1]
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+      for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          const bool reach = (dx >= 0 && dy >= 0 && dz >= 0) || (dx <= 0 && dy <= 0 && dz <= 0);
+          if (reach) {
+            const double* v = ul + 3 * (size_t)(hr + off(dx, dy, dz));
+            P[dx + 1][dy + 1][dz + 1] = make_double3(v[0], v[1], v[2]);
+          }
+        }
+    double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {  // the row is local vertex m of tet q of its hex
+        int d[4][3];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) d[v][k] = 0;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) {
+            if (l < v) d[v][kKuhnPath[q][l]] += 1;
+            if (l < m) d[v][kKuhnPath[q][l]] -= 1;
+          }
+        const double w = W[6 * (hr + off(d[0][0], d[0][1], d[0][2])) + tp[q]];
+        double G[3][3];  // G[i][j] = d u_i / d x_j
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          const int c = kKuhnPath[q][l];
+          const double3 a = P[d[l][0] + 1][d[l][1] + 1][d[l][2] + 1];
+          const double3 b = P[d[l + 1][0] + 1][d[l + 1][1] + 1][d[l + 1][2] + 1];
+          G[0][c] = (b.x - a.x) * ih[c];
+          G[1][c] = (b.y - a.y) * ih[c];
+          G[2][c] = (b.z - a.z) * ih[c];
+        }
+        const double tr = G[0][0] + G[1][1] + G[2][2];
+        double S[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) S[i][i] = fma(2.0 * mu, G[i][i], lam * tr);
+        S[0][1] = S[1][0] = mu * (G[0][1] + G[1][0]);
+        S[0][2] = S[2][0] = mu * (G[0][2] + G[2][0]);
+        S[1][2] = S[2][1] = mu * (G[1][2] + G[2][1]);
+        double f[3] = {0.0, 0.0, 0.0};
+        if (m > 0) {
+          const int c = kKuhnPath[q][m - 1];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) f[i] = S[i][c] * ih[c];
+        }
+        if (m < 3) {
+          const int c = kKuhnPath[q][m];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) f[i] = fma(-S[i][c], ih[c], f[i]);
+        }
+        const double vw = vol * w;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) y[i] = fma(vw, f[i], y[i]);
+      }
+    }
+    const int k = sub_load ? A.lidx[row] : -1;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      A.R[3 * (size_t)row + i] = y[i] - (k >= 0 ? A.load_f[3 * (size_t)k + i] : 0.0);
+  }
+}
+
 }  // namespace pa

@@ -54,6 +54,10 @@ struct BoxArgs {
   int nx, ny, nz;          // hexes per axis
   int seg_cap;             // k_box_jac_ws: blocks of one staged tile (kBoxRows x longest row)
   Material m;
+  // k_box_res_kuhn: mesh tet index of each Kuhn path, 1 / spacing per axis, tet volume
+  int tperm[6];
+  double ih[3];
+  double vol;
 };
 
 // z (and u when ulat is set) of every node at its padded lattice position (one thread
@@ -125,6 +129,12 @@ constexpr int kKuhnPath[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2,
 // R = K u - f (f subtracted when sub_load) without forming K, in FP64, from the element
 // weights A.W and u at the lattice points (A.ulat); the rows sum in a fixed order.
 __global__ void k_box_res(BoxArgs A, int sub_load);
+
+// The same residual tet by tet on lattices split into the six main-diagonal Kuhn tets
+// (see k_boxmf_apply_kuhn), all in FP64: one row per lane, u at the 15 lattice points its
+// 24 tets reach (A.ulat zero on the ghost layer and one padded plane and row past the
+// end), each tet's force vol w sigma g_m added in a fixed order.
+__global__ void k_box_res_kuhn(BoxArgs A, int sub_load);
 
 // Inverted 3x3 diagonal blocks of K(z) (the block-Jacobi preconditioner; constrained
 // dofs identity) from the element weights A.W: K_rr = -sum_{c != r} K_rc summed over

@@ -211,6 +211,10 @@ struct pa_problem {
   // Kuhn-path order (box_corners_kuhn: the corner sets in kKuhnPath order)
   bool box_kuhn = false;
   DevArray<uint8_t> box_corners_kuhn;
+  // k_box_res_kuhn in place of k_box_res on the same lattices (PA_BOX_RES_KUHN=0 keeps
+  // k_box_res); mesh tet index of each Kuhn path
+  bool box_res_kuhn = false;
+  int box_tperm[6] = {0, 0, 0, 0, 0, 0};
   double box_step[3] = {0.0, 0.0, 0.0};
   // k_box_jac_ws (store and compute waves specialised; PA_BOX_WS=0 selects k_box_jac)
   bool box_ws = true;
@@ -585,9 +589,14 @@ static int residual(pa_handle h, const double* u, const double* z, double* R, bo
   if (h->box_fast && h->box_res) {
     const pa::BoxArgs a = box_args(h, u, z, nullptr, R);
     box_weights(h, a, a.W);
-    const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
-    const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_res);
-    pa::k_box_res<<<G, 256, 0, h->stream>>>(a, sub ? 1 : 0);
+    if (h->box_res_kuhn) {
+      const int G = std::min(4096, ((h->nn + 255) / 256 + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd);
+      pa::k_box_res_kuhn<<<G, 256, 0, h->stream>>>(a, sub ? 1 : 0);
+    } else {
+      const int nblk = (h->nn + pa::kBoxRows - 1) / pa::kBoxRows;
+      const int G = std::min((nblk + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd, h->box_grid_res);
+      pa::k_box_res<<<G, 256, 0, h->stream>>>(a, sub ? 1 : 0);
+    }
     PA_HIP(hipGetLastError());
     return PA_OK;
   }
@@ -1445,7 +1454,15 @@ static int box_fast_setup(pa_handle h) {
   PA_HIP(h->box_P.upload(P));
   PA_HIP(h->box_S.upload(std::vector<pa::BoxStencil>{st}));
   PA_HIP(h->box_W.alloc((size_t)6 * (hx + 2) * (hy + 2) * (hz + 2)));
-  PA_HIP(h->box_ulat.alloc((size_t)3 * (hx + 2) * (hy + 2) * (hz + 2)));
+  {
+    // u at the lattice points, zero on the ghost layer (k_box_scatter writes only the
+    // nodes) and on a tail of one padded plane and row past the end, which
+    // k_box_res_kuhn reads at weight zero
+    const size_t npts = (size_t)(hx + 2) * (hy + 2) * (hz + 2), tail = (size_t)(hx + 2) * (hy + 2) + (hx + 2) + 1;
+    PA_HIP(h->box_ulat.alloc(3 * (npts + tail)));
+    pa::k_fill<<<grid_for(h->box_ulat.n), 256, 0, h->stream>>>(h->box_ulat.p, h->box_ulat.n, 0.0);
+    PA_HIP(hipGetLastError());
+  }
   PA_HIP(h->box_zlat.alloc((size_t)(hx + 2) * (hy + 2) * (hz + 2)));
   {  // persistent grids: the resident work-groups per CU of each variant, times the CUs
     int dev = 0, cus = 256, bk = 4, br = 3;
@@ -1502,16 +1519,21 @@ static int box_fast_setup(pa_handle h) {
   {
     // the six tets of every hex are the main-diagonal Kuhn paths (sorted corner sets
     // 0, e_a, e_a + e_b, 7)
-    h->box_kuhn = true;
+    bool kuhn = true;
     std::vector<uint8_t> ck(6 * 4);
-    for (int q = 0; q < 6 && h->box_kuhn; ++q) {
+    for (int q = 0; q < 6 && kuhn; ++q) {
       const int a = pa::kKuhnPath[q][0], b = pa::kKuhnPath[q][1];
       const std::array<uint8_t, 4> want = {0, (uint8_t)(1 << a), (uint8_t)((1 << a) | (1 << b)), 7};
-      if (std::find(types.begin(), types.end(), want) == types.end()) h->box_kuhn = false;
+      const auto it = std::find(types.begin(), types.end(), want);
+      if (it == types.end()) kuhn = false;
+      else h->box_tperm[q] = (int)(it - types.begin());
       for (int v = 0; v < 4; ++v) ck[4 * q + v] = want[v];
     }
+    h->box_kuhn = h->box_res_kuhn = kuhn;
     if (const char* e = std::getenv("PA_BOX_MF_KUHN"))
       if (std::atoi(e) == 0) h->box_kuhn = false;
+    if (const char* e = std::getenv("PA_BOX_RES_KUHN"))
+      if (std::atoi(e) == 0) h->box_res_kuhn = false;
     if (h->box_kuhn) PA_HIP(h->box_corners_kuhn.upload(ck));
     for (int k = 0; k < 3; ++k) h->box_step[k] = step[k];
   }
@@ -1540,6 +1562,9 @@ static pa::BoxArgs box_args(pa_handle h, const double* u, const double* z, doubl
   a.S = h->box_S.p;
   a.u = u;
   a.ulat = R ? h->box_ulat.p : nullptr;
+  for (int q = 0; q < 6; ++q) a.tperm[q] = h->box_tperm[q];
+  for (int k = 0; k < 3; ++k) a.ih[k] = 1.0 / h->box_step[k];
+  a.vol = h->box_step[0] * h->box_step[1] * h->box_step[2] / 6.0;
   a.zlat = h->box_zlat.p;
   a.lidx = h->box_lidx.p;
   a.load_f = h->load_f.p;
