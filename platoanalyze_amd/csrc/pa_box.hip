@@ -986,6 +986,43 @@ __global__ __launch_bounds__(256) void k_boxmf_apply_kuhn(BoxMfArgs A) {
   if (threadIdx.x == 0) A.partials[blockIdx.x] = t;
 }
 
+__global__ __launch_bounds__(256) void k_box_diag_inv_kuhn(BoxArgs A, const uint8_t* bcmask, double* dinv) {
+  const int px = A.nx + 2, pxy = (A.nx + 2) * (A.ny + 2);
+  const double ih[3] = {A.ih[0], A.ih[1], A.ih[2]};
+  const double lm = A.m.c01 + A.m.c33, mu = A.m.c33, vol = A.vol;
+  int tp[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) tp[q] = A.tperm[q];
+  const XcdSpan sp = xcd_span(A.nrows);
+  for (int row = sp.lo + sp.j * 256 + (int)threadIdx.x; row < sp.hi; row += sp.bx * 256) {
+    const int gp = A.gpack[row];
+    const int hr = ((gp & 1023) + 1) + px * ((((gp >> 10) & 1023) + 1) + (A.ny + 2) * ((gp >> 20) + 1));
+    double s[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) s[k] = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {  // the row is local vertex m of tet q of its hex
+        int d0[3] = {0, 0, 0};       // the hex's lower corner relative to the row point
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          if (l < m) d0[kKuhnPath[q][l]] -= 1;
+        const double w = A.W[6 * (hr + d0[0] + px * d0[1] + pxy * d0[2]) + tp[q]];
+        double g[3] = {0.0, 0.0, 0.0};
+        if (m > 0) g[kKuhnPath[q][m - 1]] += ih[kKuhnPath[q][m - 1]];
+        if (m < 3) g[kKuhnPath[q][m]] -= ih[kKuhnPath[q][m]];
+        const double vw = vol * w, gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) s[3 * a + b] = fma(vw, lm * g[a] * g[b] + (a == b ? mu * gg : 0.0), s[3 * a + b]);
+      }
+    }
+    block_inv3_masked(s, bcmask[row], dinv + 9 * (size_t)row);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_box_res_kuhn(BoxArgs A, int sub_load) {
   const int px = A.nx + 2, pxy = (A.nx + 2) * (A.ny + 2);
   const double ih[3] = {A.ih[0], A.ih[1], A.ih[2]};

@@ -136,6 +136,10 @@ __global__ void k_box_res(BoxArgs A, int sub_load);
 // end), each tet's force vol w sigma g_m added in a fixed order.
 __global__ void k_box_res_kuhn(BoxArgs A, int sub_load);
 
+// Inverted 3x3 diagonal blocks from the same Kuhn tets, in FP64: K_rr = sum over the
+// row's 24 tets of vol w ((lambda + mu) g g^T + mu |g|^2 I), g the row vertex's gradient.
+__global__ void k_box_diag_inv_kuhn(BoxArgs A, const uint8_t* bcmask, double* dinv);
+
 // Inverted 3x3 diagonal blocks of K(z) (the block-Jacobi preconditioner; constrained
 // dofs identity) from the element weights A.W: K_rr = -sum_{c != r} K_rc summed over
 // the row's lanes in a fixed order.

@@ -211,8 +211,9 @@ struct pa_problem {
   // Kuhn-path order (box_corners_kuhn: the corner sets in kKuhnPath order)
   bool box_kuhn = false;
   DevArray<uint8_t> box_corners_kuhn;
-  // k_box_res_kuhn in place of k_box_res on the same lattices (PA_BOX_RES_KUHN=0 keeps
-  // k_box_res); mesh tet index of each Kuhn path
+  // k_box_res_kuhn and k_box_diag_inv_kuhn in place of k_box_res and k_box_diag_inv on
+  // the same lattices (PA_BOX_RES_KUHN=0 keeps the stencil kernels); mesh tet index of
+  // each Kuhn path
   bool box_res_kuhn = false;
   int box_tperm[6] = {0, 0, 0, 0, 0, 0};
   double box_step[3] = {0.0, 0.0, 0.0};
@@ -1025,7 +1026,11 @@ static int mixed_setup(pa_handle h, const double* z) {
     if (h->box_kuhn) a32.corners = h->box_corners_kuhn.p;
     box_weights(h, a32, h->box_W32.p);
     box_weights(h, a, a.W);
-    pa::k_box_diag_inv<<<grid_for(((size_t)nn + 31) / 32, 1, 16384), 256, 0, s>>>(a, h->bcmask.p, h->m_dinv.p);
+    if (h->box_res_kuhn)
+      pa::k_box_diag_inv_kuhn<<<std::min(4096, ((nn + 255) / 256 + pa::kNumXcd - 1) / pa::kNumXcd * pa::kNumXcd), 256,
+                                0, s>>>(a, h->bcmask.p, h->m_dinv.p);
+    else
+      pa::k_box_diag_inv<<<grid_for(((size_t)nn + 31) / 32, 1, 16384), 256, 0, s>>>(a, h->bcmask.p, h->m_dinv.p);
     PA_HIP(hipGetLastError());
   } else {
     PA_HIP(pa::launch_mf_diag_inv(da, s));
