@@ -1467,6 +1467,7 @@ static int box_fast_setup(pa_handle h) {
     PA_HIP(h->box_ulat.alloc(3 * (npts + tail)));
     pa::k_fill<<<grid_for(h->box_ulat.n), 256, 0, h->stream>>>(h->box_ulat.p, h->box_ulat.n, 0.0);
     PA_HIP(hipGetLastError());
+    PA_HIP(hipStreamSynchronize(h->stream));  // done before any later stream (pa_set_stream) writes u
   }
   PA_HIP(h->box_zlat.alloc((size_t)(hx + 2) * (hy + 2) * (hz + 2)));
   {  // persistent grids: the resident work-groups per CU of each variant, times the CUs
